@@ -1,0 +1,337 @@
+/*
+ * millefeuille.h -- C ABI of libmillefeuille_hip.so, the MI355X (gfx950)
+ * tiled SpMV and single-kernel CG / BiCGSTAB library.
+ *
+ * Conventions
+ *   - Every entry point returns an mf_status: 0 = OK, negative = error,
+ *     positive = ran but did not converge.  mf_last_error() returns a
+ *     thread-local message for the last failure on the calling thread.
+ *   - Plain pointers and sizes only.  Pointers named d_* are device pointers
+ *     (HBM, allocated by the caller on the handle's device); everything else
+ *     is host memory.
+ *   - Streams are passed as void* (a hipStream_t; NULL = the null stream).
+ *   - One handle may be used by one host thread at a time.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef MILLEFEUILLE_H
+#define MILLEFEUILLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int mf_status;
+
+#define MF_OK 0
+#define MF_NOT_CONVERGED 1
+#define MF_ERR_ARG (-1)
+#define MF_ERR_IO (-2)
+#define MF_ERR_FORMAT (-3)
+#define MF_ERR_HIP (-4)
+#define MF_ERR_NOMEM (-5)
+#define MF_ERR_UNSUPPORTED (-6)
+#define MF_ERR_BREAKDOWN (-7)
+#define MF_ERR_COOP (-8)
+
+/* Per-tile precision levels.  Reference flags (src/main-cg-mixed.cu:236-322,
+ * d_vis): 0 = FP64, 4 = FP32, 3 = FP16, 2 = INT8, 1 = bypass. */
+#define MF_PREC_FP64 0
+#define MF_PREC_FP32 1
+#define MF_PREC_FP16 2
+#define MF_PREC_INT8 3
+#define MF_PREC_SKIP 4
+
+/* Host CSR matrix.  Arrays owned by the library when filled by mf_read_mtx
+ * (release with mf_csr_free). */
+typedef struct mf_csr {
+  int m, n;
+  int64_t nnz;
+  int *rowptr; /* m+1 */
+  int *colidx; /* nnz */
+  double *val; /* nnz */
+  int is_sym;  /* banner said symmetric/hermitian (entries already mirrored) */
+} mf_csr;
+
+/* Reads a Matrix Market coordinate file into CSR in file order, mirroring
+ * symmetric/hermitian entries.
+ * Replaces read_Dmatrix_32 (src/biio2.0/src/biio.h:153-194) with
+ * mmio_info/mmio_data (src/biio2.0/src/mmio_highlevel.h:7-321). */
+mf_status mf_read_mtx(const char *path, mf_csr *out);
+/* Binary CSR cache: int32 rows, int32 cols, int64 nnz, int64 rowptr[rows+1],
+ * int32 col[nnz], double val[nnz].
+ * Replaces binary_read_csr / binary_write_csr (src/biio2.0/src/biio.h:22-97). */
+mf_status mf_read_cbd(const char *path, mf_csr *out);
+mf_status mf_write_cbd(const char *path, const mf_csr *a);
+void mf_csr_free(mf_csr *a);
+
+/* ---------------------------------------------------------------------- */
+/* Tiled matrix handle                                                     */
+/* ---------------------------------------------------------------------- */
+
+typedef struct mf_tiled mf_tiled;
+
+/* Builds the 16x16 tiled CSR of rows [0,rowA) and columns [0,colA) of the
+ * CSR (rowptr has at least rowA+1 entries; val_low may be NULL, then the FP32
+ * copy is (float)val), uploads the device layout to `device` and packs the
+ * FP32 / FP16 / INT8 payloads there.  device = -1 builds a host-only handle
+ * (the reference arrays and preprocessing, no HIP calls); device entry points
+ * reject such a handle with MF_ERR_ARG.
+ * Replaces Tile_create (src/csr2block.h:267-422) plus the H2D phase of the
+ * solvers (src/main-cg.cu:441-516, src/main-cg-mixed.cu:2299-2362). */
+mf_status mf_tile_create(int rowA, int colA, const int *rowptr,
+                         const int *colidx, const double *val,
+                         const float *val_low, int device, mf_tiled **out);
+void mf_tile_free(mf_tiled *t);
+
+/* Shapes: tilem, tilen, tilenum, csrsize (nnz), csrptrlen, npad (length of
+ * padded device vectors = 16*max(tilem,tilen)). */
+mf_status mf_tile_info(const mf_tiled *t, int *tilem, int *tilen,
+                       int *tilenum, int *csrsize, int *csrptrlen, int *npad);
+
+/* Host views of the reference-identical Tile_create arrays (owned by the
+ * handle; valid until mf_tile_free).  Sizes: tile_ptr tilem+1,
+ * tile_columnidx tilenum, tile_nnz/blknnz/csr_offset/csrptr_offset
+ * tilenum+1, blknnznnz tilenum+1, Blockcsr_Val/_Low/Tile_csr_Col csrsize,
+ * Blockcsr_Ptr csrptrlen.  Field meanings: src/format.h:19-45. */
+typedef struct mf_tile_arrays {
+  const int *tile_ptr, *tile_columnidx, *tile_nnz, *blknnz, *csr_offset,
+      *csrptr_offset;
+  const unsigned char *blknnznnz, *Tile_csr_Col, *Blockcsr_Ptr;
+  const double *Blockcsr_Val;
+  const float *Blockcsr_Val_Low;
+} mf_tile_arrays;
+mf_status mf_tile_arrays_get(const mf_tiled *t, mf_tile_arrays *out);
+
+/* Reference-derived host arrays, for callers that drive the reference's own
+ * kernels or its CPU check loop.  All output arrays are caller-allocated.
+ *   mf_tile_offsets: ptroffset1/ptroffset2 (tilenum each) and the chunked
+ *     row-block list (rowblkblock entries, see mf_tile_rowblkblock);
+ *     replaces blockspmv_cpu's preprocessing (src/blockspmv_cpu.h:2-83).
+ *   mf_tile_compact: nonzero_row_new (tilenum+1), then blockrowid_new /
+ *     blockcsr_ptr_new (cnt+1), block_signal (tilem+1), tile_rowidx
+ *     (tilenum); replaces src/main-cg.cu:536-593.
+ *   mf_tile_balance: the reference's two-pointer nnz balancer, returns the
+ *     warp count in *nwarps; balance_tile_ptr (nwarps+1, sized tilenum+2 is
+ *     always enough), index_each_block / row_each_block (tilenum+1);
+ *     replaces src/main-cg.cu:596-761. */
+mf_status mf_tile_rowblkblock(const mf_tiled *t, int *rowblkblock);
+mf_status mf_tile_offsets(const mf_tiled *t, int *ptroffset1, int *ptroffset2,
+                          unsigned int *blkcoostylerowidx, int *colstart,
+                          int *colstop);
+mf_status mf_tile_compact_count(const mf_tiled *t, int *nonzero_row_new,
+                                int *cnt_non_new);
+mf_status mf_tile_compact_fill(const mf_tiled *t, const int *nonzero_row_new,
+                               unsigned char *blockrowid_new,
+                               unsigned char *blockcsr_ptr_new,
+                               int *block_signal, int *tile_rowidx);
+mf_status mf_tile_balance(const mf_tiled *t, int warp_limit,
+                          int *balance_tile_ptr, int *index_each_block,
+                          int *row_each_block, int *nwarps);
+
+/* Nibble codec for tile-local column indices, two per byte, even position in
+ * the high nibble (src/encode.h:29-72). */
+void mf_encode_nibbles(const unsigned char *cols, unsigned char *packed,
+                       int n, int offset);
+void mf_decode_nibbles(const unsigned char *packed, unsigned char *cols, int n);
+
+/* ---------------------------------------------------------------------- */
+/* Per-tile precision                                                      */
+/* ---------------------------------------------------------------------- */
+
+/* Sets every tile's precision level (MF_PREC_*) from a host array of tilenum
+ * bytes, or all tiles to `uniform` when levels is NULL. */
+mf_status mf_tile_set_precision(mf_tiled *t, const unsigned char *levels,
+                                int uniform);
+mf_status mf_tile_get_precision(const mf_tiled *t, unsigned char *levels);
+/* Per-tile payload scales (tilenum floats each; FP16 power-of-two scale and
+ * INT8 max/127 scale) and per-tile max |a - dequant(a)| for FP32/FP16/INT8
+ * (3*tilenum doubles, level-major).  Host copies. */
+mf_status mf_tile_get_scales(const mf_tiled *t, float *scale16, float *scale8);
+mf_status mf_tile_get_quant_error(const mf_tiled *t, double *err);
+
+/* ---------------------------------------------------------------------- */
+/* SpMV                                                                    */
+/* ---------------------------------------------------------------------- */
+
+/* y = A x on the device.  d_x holds npad doubles (zero beyond colA), d_y
+ * receives npad doubles (rows beyond rowA are zero).  mode: 0 = all tiles
+ * FP64; 1 = per-tile precision levels set on the handle, low-precision tiles
+ * multiply the FP32 copy of x (d_x32, npad floats; required for mode 1) and
+ * accumulate in FP32 per tile.
+ * Replaces stir_spmv_cuda_kernel_newcsr (src/Mille-feuille-cg.h:97-166) and
+ * the tile loop of blockspmv_cpu (src/blockspmv_cpu.h:85-112). */
+mf_status mf_spmv(mf_tiled *t, const double *d_x, const float *d_x32,
+                  double *d_y, int mode, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Solvers (single persistent kernel per solve)                            */
+/* ---------------------------------------------------------------------- */
+
+/* Precision policies for the mixed solvers. */
+#define MF_POLICY_FP64 0    /* every tile FP64: the plain single-kernel CG */
+#define MF_POLICY_RULE 1    /* reference rule on |p_new - p_old| per segment */
+#define MF_POLICY_FIXED 2   /* use the levels set with mf_tile_set_precision */
+#define MF_POLICY_LEARNED 3 /* MLP over per-tile features (mf_policy_*) */
+#define MF_POLICY_ERRBOUND 4 /* analytic per-tile quantisation error bound */
+
+typedef struct mf_solve_opts {
+  double tol;           /* stop at ||b - A x||_2 <= tol * ||b||_2 (FP64) */
+  int maxiter;          /* iteration cap (0 -> 10000) */
+  int policy;           /* MF_POLICY_* */
+  int check_interval;   /* iterations between policy evaluations (0 -> 8) */
+  double refresh_ratio; /* recompute r = b - A x in FP64 when the recursive
+                           residual fell by this factor (0 -> 0.1) */
+  int max_refresh;      /* refinement steps before FP64 fallback (0 -> 64) */
+  int refresh_every;    /* mixed policies: FP64 residual check at least every
+                           this many iterations; no progress since the last
+                           check -> FP64 fallback and restart (0 -> 64) */
+  int blocks;           /* persistent grid size override (0 = occupancy) */
+  int fixed_iters;      /* >0: run exactly this many iterations, no test
+                           (reference timing mode, src/Mille-feuille-cg.h:426) */
+  void *stream;         /* hipStream_t */
+} mf_solve_opts;
+
+typedef struct mf_report {
+  int iterations;       /* CG / BiCGSTAB iterations executed */
+  int refreshes;        /* FP64 true-residual recomputations */
+  int fell_back;        /* 1 if the solver promoted every tile to FP64 */
+  int status;           /* same as the return value */
+  double rel_residual;  /* device FP64 ||b - A x|| / ||b|| at exit */
+  double rel_residual_recursive; /* recursive residual at exit */
+  double time_ms;       /* kernel time (HIP events) */
+  long long prec_hist[5]; /* tile-iterations per precision level */
+  int grid_blocks;      /* persistent grid actually launched */
+} mf_report;
+
+void mf_solve_opts_default(mf_solve_opts *o);
+
+/* Solves A x = b by CG in one persistent cooperative kernel.  d_b, d_x are
+ * npad doubles on the device; d_x holds the initial guess on entry (zero
+ * padding beyond rowA) and the solution on exit.
+ * Replaces cg_solve_sync / cg_solve_inc (src/main-cg.cu:379, 1071) and the
+ * mixed cg_solve_inc (src/main-cg-mixed.cu:2238) with their persistent
+ * kernels (src/Mille-feuille-cg.h:352-580, src/main-cg-mixed.cu:998-1377). */
+mf_status mf_cg_solve(mf_tiled *t, const double *d_b, double *d_x,
+                      const mf_solve_opts *opts, mf_report *rep);
+
+/* BiCGSTAB in one persistent cooperative kernel, same conventions.
+ * Replaces bicgstab_sync / bicgstab_inc (src/main-bicg.cu:13, 711), the
+ * mixed bicgstab (src/main-bicg-mixed.cu:1355) and their kernels
+ * (src/Mille-feuille-bicg.h:202-1185, src/main-bicg-mixed.cu:122-1353). */
+mf_status mf_bicgstab_solve(mf_tiled *t, const double *d_b, double *d_x,
+                            const mf_solve_opts *opts, mf_report *rep);
+
+/* Multi-kernel CG / BiCGSTAB: one launch per step (SpMV, dot, axpy), the
+ * host reads the residual each iteration.
+ * Replaces cg_solve_reduce (src/main-cg.cu:6-376) and bicgstab
+ * (src/main-bicg.cu:1336-1695). */
+mf_status mf_cg_solve_multikernel(mf_tiled *t, const double *d_b, double *d_x,
+                                  const mf_solve_opts *opts, mf_report *rep);
+mf_status mf_bicgstab_solve_multikernel(mf_tiled *t, const double *d_b,
+                                        double *d_x, const mf_solve_opts *opts,
+                                        mf_report *rep);
+
+/* ---------------------------------------------------------------------- */
+/* Reference-compatible entry points (host buffers in and out)            */
+/* ---------------------------------------------------------------------- */
+
+/* Same argument lists and semantics as the reference: rows truncated to a
+ * multiple of 16 (n is the truncated count, ori the original column count),
+ * x is overwritten with the solution, iter/maxiter/threshold are ignored as
+ * in the reference, a CSV row is appended to data/cg_performance.csv.
+ * Unlike the reference these iterate to the FP64 relative residual 1e-10
+ * (set MF_COMPAT_FIXED_ITERS=100 in the environment for the reference's fixed
+ * 100 iterations).  src/main-cg.cu:6, 379, 1071; src/main-cg-mixed.cu:2238. */
+void cg_solve_reduce(int *RowPtr, int *ColIdx, double *Val, float *Val_Low,
+                     double *x, double *b, int n, int *iter, int maxiter,
+                     double threshold, char *filename, int nnzR, int ori);
+void cg_solve_sync(int *RowPtr, int *ColIdx, double *Val, float *Val_Low,
+                   double *x, double *b, int n, int *iter, int maxiter,
+                   double threshold, char *filename, int nnzR, int ori);
+void cg_solve_inc(int *RowPtr, int *ColIdx, double *Val, float *Val_Low,
+                  double *x, double *b, int n, int *iter, int maxiter,
+                  double threshold, char *filename, int nnzR, int ori);
+void cg_solve_mixed(int *RowPtr, int *ColIdx, double *Val, float *Val_Low,
+                    double *x, double *b, int n, int *iter, int maxiter,
+                    double threshold, char *filename, int nnzR, int ori);
+/* rhs is overwritten with the row sums of A (A*ones) and x with the
+ * solution; a CSV row goes to data/bicg_performance.csv.
+ * src/main-bicg.cu:13, 711, 1336; src/main-bicg-mixed.cu:1355. */
+int bicgstab(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x,
+             int nnzR, char *filename, int n);
+int bicgstab_sync(int *RowPtr, int *ColIdx, double *Val, double *rhs,
+                  double *x, int nnzR, char *filename, int n);
+int bicgstab_inc(int *RowPtr, int *ColIdx, double *Val, double *rhs,
+                 double *x, int nnzR, char *filename, int n);
+int bicgstab_mixed(int *RowPtr, int *ColIdx, double *Val, double *rhs,
+                   double *x, int nnzR, char *filename, int n);
+/* src/biio2.0/src/biio.h:153 */
+void read_Dmatrix_32(int *row, int *col, int *nnz, int **row_ptr,
+                     int **col_idx, double **val, int *isSymmeticeR,
+                     char *filename);
+
+/* ---------------------------------------------------------------------- */
+/* Learned tile-precision policy (no reference counterpart: the reference */
+/* has only the rule of src/main-cg-mixed.cu:1316-1364)                    */
+/* ---------------------------------------------------------------------- */
+
+#define MF_MLP_IN 8
+#define MF_MLP_H1 32
+#define MF_MLP_H2 32
+#define MF_MLP_OUT 4
+#define MF_MLP_NPARAM                                                     \
+  (MF_MLP_H1 * MF_MLP_IN + MF_MLP_H1 + MF_MLP_H2 * MF_MLP_H1 + MF_MLP_H2 + \
+   MF_MLP_OUT * MF_MLP_H2 + MF_MLP_OUT)
+
+/* MLP 8->32->32->4 with ReLU; parameters packed as w1[32][8], b1[32],
+ * w2[32][32], b2[32], w3[4][32], b3[4] (1476 values).  The device copy is
+ * FP16 with FP32 accumulation; training keeps an FP32 master copy.
+ *   mf_mlp_forward: logits (nrows x 4 floats) for features (nrows x 8).
+ *   mf_mlp_grad:    mean softmax cross-entropy loss and its gradient with
+ *                   respect to the FP32 parameters (all device pointers).
+ *   mf_mlp_train:   epochs of minibatch Adam on the device; returns the
+ *                   final loss. */
+mf_status mf_mlp_forward(const float *d_params, const float *d_feat, int nrows,
+                         float *d_logits, void *stream);
+mf_status mf_mlp_grad(const float *d_params, const float *d_feat,
+                      const int *d_label, int nrows, float *d_grad,
+                      float *d_loss, void *stream);
+mf_status mf_mlp_train(float *d_params, const float *d_feat,
+                       const int *d_label, int nrows, int epochs, int batch,
+                       float lr, unsigned seed, float *loss_out, void *stream);
+/* Installs trained parameters (device FP32, converted to FP16 inside) as the
+ * handle's learned policy. */
+mf_status mf_policy_set_mlp(mf_tiled *t, const float *d_params);
+/* Per-tile features the policy sees (tilenum x 8 floats, device), computed
+ * for a given p vector, residual norm ratio and iteration. */
+mf_status mf_policy_features(mf_tiled *t, const double *d_p, double res_ratio,
+                             int iteration, float *d_feat, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Batched solves                                                          */
+/* ---------------------------------------------------------------------- */
+
+/* Solves many independent SPD systems, each with nrhs right-hand sides, by
+ * CG in one persistent launch captured in a hipGraph (FP32 / FP16 tiles,
+ * FP64 scalars and residual refresh).  systems[i] are tiled handles on the
+ * same device; d_b[i] / d_x[i] point to npad_i * nrhs doubles (column-major
+ * by right-hand side).  per_system_ms (optional, host, nsys entries) receives
+ * each system's solve time as completion offsets within the launch.
+ * No reference counterpart (the reference solves one system per process). */
+mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs,
+                            const double **d_b, double **d_x,
+                            const mf_solve_opts *opts, int *iterations,
+                            double *rel_residual, double *per_system_ms,
+                            double *total_ms);
+
+const char *mf_last_error(void);
+const char *mf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

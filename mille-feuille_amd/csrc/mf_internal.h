@@ -1,0 +1,81 @@
+// mf_internal.h -- shared internal declarations of libmillefeuille_hip.
+//
+// Host-only (no HIP types) so that the format builder compiles with g++.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "millefeuille.h"
+
+namespace mf {
+
+constexpr int kTile = 16;  // tile edge (reference BLOCK_SIZE, src/common.h:47-50)
+
+// Thread-local error message (mf_last_error).
+void set_error(const std::string &msg);
+mf_status fail(mf_status code, const std::string &msg);
+
+// Reference-identical Tile_create output plus the handle's device layout.
+struct DeviceLayout {
+  int device = -1;
+  int npad = 0;                // padded vector length = 16 * max(tilem, tilen)
+  int *tile_ptr = nullptr;     // tilem+1
+  int *tile_col = nullptr;     // tilenum
+  int *tile_off = nullptr;     // tilenum+1 (payload offset; == csr_offset)
+  int *tile_row = nullptr;     // tilenum (tile-row of each tile)
+  uint8_t *rowptr = nullptr;   // tilenum*16 row starts (rows >= h start at nnz)
+  uint8_t *col = nullptr;      // csrsize local columns
+  double *v64 = nullptr;       // csrsize FP64 payload
+  float *v32 = nullptr;        // csrsize FP32 payload
+  uint16_t *v16 = nullptr;     // csrsize FP16 payload (bits), scaled by s16
+  int8_t *v8 = nullptr;        // csrsize INT8 payload, scaled by s8
+  float *s16 = nullptr;        // tilenum power-of-two scales
+  float *s8 = nullptr;         // tilenum max/127 scales
+  double *qerr = nullptr;      // 3*tilenum max |a - dequant_p(a)|, p=FP32,FP16,INT8
+  double *amax = nullptr;      // tilenum max |a|
+  uint8_t *prec = nullptr;     // tilenum current precision levels
+  uint8_t *prec_user = nullptr;// tilenum levels set by mf_tile_set_precision
+  int *diag_tile = nullptr;    // tilem: index of the diagonal tile or -1
+  // learned policy parameters (FP16 bits), MF_MLP_NPARAM
+  uint16_t *mlp16 = nullptr;
+  int has_mlp = 0;
+};
+
+struct Schedule {
+  // Contiguous tile-row ranges, one per persistent wave, balanced on nnz.
+  int nwaves = 0;
+  int *wave_tr = nullptr;  // device, nwaves+1 tile-row bounds
+};
+
+}  // namespace mf
+
+struct mf_tiled {
+  int rowA = 0, colA = 0, tilem = 0, tilen = 0, tilenum = 0;
+  int csrsize = 0, csrptrlen = 0;
+  std::vector<int> tile_ptr, tile_columnidx, tile_nnz, blknnz, csr_offset,
+      csrptr_offset;
+  std::vector<unsigned char> blknnznnz, Tile_csr_Col, Blockcsr_Ptr;
+  std::vector<double> Blockcsr_Val;
+  std::vector<float> Blockcsr_Val_Low;
+  mf::DeviceLayout dev;
+  mf::Schedule sched;
+  int sched_waves_for = 0;  // wave count the cached schedule was built for
+  // device scratch owned by the handle (solver workspace), grown on demand
+  void *work = nullptr;
+  size_t work_bytes = 0;
+};
+
+namespace mf {
+// Host format build (mf_format.cpp).
+mf_status build_tiles_host(mf_tiled *t, int rowA, int colA, const int *rowptr,
+                           const int *colidx, const double *val,
+                           const float *val_low);
+// Device upload + payload packing (mf_device.hip).
+mf_status upload_tiles(mf_tiled *t, int device);
+void free_device(mf_tiled *t);
+// nnz-balanced contiguous tile-row partition for `nwaves` waves.
+std::vector<int> balance_tilerows(const mf_tiled *t, int nwaves);
+}  // namespace mf

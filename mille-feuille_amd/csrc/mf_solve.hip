@@ -1,0 +1,722 @@
+// mf_solve.hip -- single-kernel (persistent, cooperative) CG and BiCGSTAB.
+//
+// One launch runs the whole solve.  The grid is sized to the occupancy of
+// the kernel on every CU and launched cooperatively, so all blocks are
+// co-resident and the epoch grid barrier cannot deadlock (the reference
+// assumes co-residency and hardcodes the A100 warp count,
+// src/main-cg.cu:675-696).  Every wave owns a contiguous, nnz-balanced range
+// of tile-rows: it computes the SpMV rows of those tile-rows (lanes 0..15 hold
+// the 16 row sums) and does the vector updates of the same rows, so the
+// vector phases never read data written by another wave in the same phase.
+// Dot products go through deterministic grid sums: the result is bitwise
+// identical run to run and in every block (the reference's double atomics
+// are order dependent, src/Mille-feuille-cg.h:489-509).
+//
+// FP64 is kept for the scalars (alpha, beta, omega), the solution update and
+// the true residual r = b - A x, which is recomputed in FP64 whenever the
+// recursive residual meets the tolerance (so the exit test is always on the
+// FP64 residual) and, for the mixed policies, whenever the recursive residual
+// fell by `refresh_ratio` since the last recomputation (reliable updates,
+// i.e. iterative refinement inside one Krylov run).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mf_device.h"
+
+namespace mf {
+
+constexpr int kBlock = 1024;  // 16 waves, one block per CU
+constexpr int kWavesPerBlock = kBlock / 64;
+
+enum StatSlot {
+  kStIters = 0,
+  kStRefresh,
+  kStFallback,
+  kStRecursive,
+  kStTrue,
+  kStBB,
+  kStStatus,
+  kStCount = 16
+};
+
+struct SolveArgs {
+  TileView A;
+  const double *b;
+  double *x;
+  double *r, *p, *q;          // CG (q doubles as v in BiCGSTAB)
+  double *rh, *s, *tv;        // BiCGSTAB
+  float *p32, *s32;
+  uint8_t *seg_p, *seg_s;     // per-segment levels (rule policy)
+  const int *wave_tr;
+  double *partials;
+  GridSync *gs;
+  double *stats;
+  unsigned long long *hist;   // 5 counters
+  double tol2;                // tol^2
+  int maxiter;
+  int fixed_iters;
+  int policy;
+  int check_interval;
+  double ratio2;              // refresh_ratio^2
+  int max_refresh;
+  int refresh_every;
+};
+
+// Reference rule (src/main-cg-mixed.cu:1326-1363): level from the absolute
+// change of a 16-entry segment of the search direction.
+__device__ __forceinline__ int rule_level(double d) {
+  if (d > 1e-5) return MF_PREC_FP64;
+  if (d > 1e-6) return MF_PREC_FP32;
+  if (d > 1e-7) return MF_PREC_FP16;
+  if (d > 1e-8) return MF_PREC_INT8;
+  return MF_PREC_SKIP;
+}
+
+// Max over the 16 lanes of each group.
+__device__ __forceinline__ double group16_max(double v) {
+#pragma unroll
+  for (int m = 8; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m));
+  return v;
+}
+
+__device__ __forceinline__ void flush_hist(const PrecCount &c, unsigned long long *hist,
+                                           double *lds) {
+  double v0 = c.c0, v1 = c.c1, v2 = c.c2, v3 = c.c3, v4 = c.c4;
+  v0 = block_sum(v0, lds);
+  v1 = block_sum(v1, lds);
+  v2 = block_sum(v2, lds);
+  v3 = block_sum(v3, lds);
+  v4 = block_sum(v4, lds);
+  if (threadIdx.x == 0) {
+    atomicAdd(&hist[0], (unsigned long long)v0);
+    atomicAdd(&hist[1], (unsigned long long)v1);
+    atomicAdd(&hist[2], (unsigned long long)v2);
+    atomicAdd(&hist[3], (unsigned long long)v3);
+    atomicAdd(&hist[4], (unsigned long long)v4);
+  }
+}
+
+// True residual of the wave's tile-rows: r_new = b - A x (FP64), returned
+// per lane (lanes 0..15) and optionally stored.  Accumulates ||r_new||^2.
+__device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int tr0, int tr1,
+                                                     int lane, double *store) {
+  double acc = 0.0;
+  for (int bi = tr0; bi < tr1; ++bi) {
+    const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
+    if (lane < 16) {
+      const int i = bi * 16 + lane;
+      const double rv = a.b[i] - ax;
+      if (store) store[i] = rv;
+      acc += rv * rv;
+    }
+  }
+  return acc;
+}
+
+// Restart helper: r = b - A x in FP64 for the wave's tile-rows, p = r (and
+// r^ = r for BiCGSTAB).  Returns the wave's share of ||r||^2.
+__device__ __forceinline__ double restart_rows(const SolveArgs &a, int tr0, int tr1, int lane,
+                                               bool mixed, bool bicg) {
+  double acc = 0.0;
+  for (int bi = tr0; bi < tr1; ++bi) {
+    const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
+    if (lane < 16) {
+      const int i = bi * 16 + lane;
+      const double rv = a.b[i] - ax;
+      a.r[i] = rv;
+      a.p[i] = rv;
+      if (mixed) a.p32[i] = (float)rv;
+      if (bicg) a.rh[i] = rv;
+      acc += rv * rv;
+    }
+  }
+  return acc;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
+  __shared__ double lds[kWavesPerBlock];
+  __shared__ int lds_i;
+  const int lane = threadIdx.x & 63;
+  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int tr0 = a.wave_tr[wid], tr1 = a.wave_tr[wid + 1];
+  const int row0 = tr0 * 16, row1 = tr1 * 16;
+  constexpr bool mixed = MODE != kAll64;
+  unsigned epoch = 0;
+  int slot = 0;
+  PrecCount cnt;
+
+  // r = b - A x0, p = r.
+  double red[2] = {0.0, 0.0};
+  for (int bi = tr0; bi < tr1; ++bi) {
+    const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
+    if (lane < 16) {
+      const int i = bi * 16 + lane;
+      const double bv = a.b[i], rv = bv - ax;
+      a.r[i] = rv;
+      a.p[i] = rv;
+      if (mixed) a.p32[i] = (float)rv;
+      red[0] += rv * rv;
+      red[1] += bv * bv;
+    }
+  }
+  grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
+  double rr = red[0];
+  const double bb = red[1];
+  const double stop2 = a.tol2 * bb;
+  double rr_ref = rr;  // residual norm^2 at the last FP64 recomputation
+  int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0;
+  bool restart = false;
+  bool done = bb == 0.0 || (a.fixed_iters <= 0 && rr <= stop2);
+
+  while (!done) {
+    if (restart) {
+      // Fallback: FP64 residual, p = r, every tile FP64 from here on.
+      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, false)};
+      grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
+      ++refresh;
+      rr = tr[0];
+      rr_ref = rr;
+      it_ref = it;
+      restart = false;
+      if (rr <= stop2) break;
+    }
+    ++it;
+    // q = A p, fused with p.q for the wave's rows.
+    double pq[1] = {0.0};
+    for (int bi = tr0; bi < tr1; ++bi) {
+      const double qv = tile_row_sum<MODE, mixed>(a.A, bi, lane, a.p, a.p32, a.seg_p, force64, &cnt);
+      if (lane < 16) {
+        const int i = bi * 16 + lane;
+        a.q[i] = qv;
+        pq[0] += a.p[i] * qv;
+      }
+    }
+    grid_sum<1>(pq, a.partials, a.gs, epoch, slot, lds, &lds_i);
+    if (mixed && !force64 && !(pq[0] > 0.0 && isfinite(pq[0]))) {
+      // The low-precision operator lost positive definiteness.
+      force64 = 1;
+      restart = true;
+      continue;
+    }
+    if (!(pq[0] != 0.0) || !isfinite(pq[0])) { status = MF_ERR_BREAKDOWN; break; }
+    const double alpha = rr / pq[0];
+    // x += alpha p, r -= alpha q, ||r||^2.
+    double rn[1] = {0.0};
+    for (int i = row0 + lane; i < row1; i += 64) {
+      const double pv = a.p[i];
+      a.x[i] += alpha * pv;
+      const double rv = a.r[i] - alpha * a.q[i];
+      a.r[i] = rv;
+      rn[0] += rv * rv;
+    }
+    grid_sum<1>(rn, a.partials, a.gs, epoch, slot, lds, &lds_i);
+    double rr_new = rn[0];
+    if (a.fixed_iters > 0) {
+      if (it >= a.fixed_iters) { rr = rr_new; break; }
+    } else {
+      const bool converged = rr_new <= stop2;
+      const bool periodic = mixed && !force64 &&
+                            (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
+      if (converged || periodic) {
+        // FP64 recomputation of the residual (x is complete: the grid sum
+        // above is a barrier after the x update).
+        double tr[1] = {true_residual_rows(a, tr0, tr1, lane, a.r)};
+        grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
+        ++refresh;
+        it_ref = it;
+        if (tr[0] <= stop2) { rr = tr[0]; break; }
+        // No progress since the last recomputation, or too many refinement
+        // steps: every tile back to FP64 and restart the recurrence.
+        if (mixed && !force64 && (tr[0] > 0.25 * rr_ref || refresh > a.max_refresh)) {
+          force64 = 1;
+          restart = true;
+        }
+        rr_new = tr[0];
+        rr_ref = tr[0];
+      }
+      if (it >= a.maxiter) { rr = rr_new; status = MF_NOT_CONVERGED; break; }
+      if (restart) continue;
+    }
+    const double beta = rr_new / rr;
+    rr = rr_new;
+    // p = r + beta p; the rule policy looks at |p_new - p_old| per segment.
+    const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
+    for (int base = row0; base < row1; base += 64) {
+      const int i = base + lane;
+      double d = 0.0;
+      if (i < row1) {
+        const double po = a.p[i];
+        const double pv = a.r[i] + beta * po;
+        a.p[i] = pv;
+        if (mixed) a.p32[i] = (float)pv;
+        d = fabs(pv - po);
+      }
+      if (upd) {
+        d = group16_max(d);
+        if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
+      }
+    }
+    grid_sync(a.gs, epoch, &lds_i);
+  }
+  if (mixed) flush_hist(cnt, a.hist, lds);
+  // Final FP64 residual for the report.
+  double fin[1] = {true_residual_rows(a, tr0, tr1, lane, nullptr)};
+  grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.stats[kStIters] = it;
+    a.stats[kStRefresh] = refresh;
+    a.stats[kStFallback] = force64;
+    a.stats[kStRecursive] = rr;
+    a.stats[kStTrue] = fin[0];
+    a.stats[kStBB] = bb;
+    a.stats[kStStatus] = status;
+  }
+}
+
+// BiCGSTAB (van der Vorst) with the reference's scalar formulas
+// (src/Mille-feuille-bicg.h:358-460).  v lives in a.q, t in a.tv.  A
+// breakdown (v.r^ = 0, omega = 0 or rho = 0) or, for the mixed policies, a
+// stagnating FP64 residual restarts the recurrence from the FP64 residual
+// with r^ = r (and every tile FP64); more than kMaxRestart restarts report a
+// breakdown.
+constexpr int kMaxRestart = 8;
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
+  __shared__ double lds[kWavesPerBlock];
+  __shared__ int lds_i;
+  const int lane = threadIdx.x & 63;
+  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int tr0 = a.wave_tr[wid], tr1 = a.wave_tr[wid + 1];
+  const int row0 = tr0 * 16, row1 = tr1 * 16;
+  constexpr bool mixed = MODE != kAll64;
+  unsigned epoch = 0;
+  int slot = 0;
+  PrecCount cnt;
+  double *v = a.q, *t = a.tv;
+
+  double red[2] = {0.0, 0.0};
+  for (int bi = tr0; bi < tr1; ++bi) {
+    const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
+    if (lane < 16) {
+      const int i = bi * 16 + lane;
+      const double bv = a.b[i], rv = bv - ax;
+      a.r[i] = rv;
+      a.rh[i] = rv;
+      a.p[i] = rv;
+      if (mixed) a.p32[i] = (float)rv;
+      red[0] += rv * rv;
+      red[1] += bv * bv;
+    }
+  }
+  grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
+  double rho = red[0], rr = red[0];
+  const double bb = red[1];
+  const double stop2 = a.tol2 * bb;
+  double rr_ref = rr;
+  int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0, restarts = 0;
+  bool restart = false;
+  bool done = bb == 0.0 || (a.fixed_iters <= 0 && rr <= stop2);
+
+  while (!done) {
+    if (restart) {
+      if (++restarts > kMaxRestart) { status = MF_ERR_BREAKDOWN; break; }
+      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, true)};
+      grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
+      ++refresh;
+      rr = rho = tr[0];
+      rr_ref = rr;
+      it_ref = it;
+      restart = false;
+      if (rr <= stop2) break;
+    }
+    ++it;
+    // v = A p, v.rh
+    double vr[1] = {0.0};
+    for (int bi = tr0; bi < tr1; ++bi) {
+      const double vv = tile_row_sum<MODE, mixed>(a.A, bi, lane, a.p, a.p32, a.seg_p, force64, &cnt);
+      if (lane < 16) {
+        const int i = bi * 16 + lane;
+        v[i] = vv;
+        vr[0] += vv * a.rh[i];
+      }
+    }
+    grid_sum<1>(vr, a.partials, a.gs, epoch, slot, lds, &lds_i);
+    if (!(vr[0] != 0.0) || !isfinite(vr[0])) {
+      if (mixed) force64 = 1;
+      restart = true;
+      continue;
+    }
+    const double alpha = rho / vr[0];
+    // s = r - alpha v (rule policy: |s_new - s_old| per segment)
+    const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
+    for (int base = row0; base < row1; base += 64) {
+      const int i = base + lane;
+      double d = 0.0;
+      if (i < row1) {
+        const double so = a.s[i];
+        const double sv = a.r[i] - alpha * v[i];
+        a.s[i] = sv;
+        if (mixed) a.s32[i] = (float)sv;
+        d = fabs(sv - so);
+      }
+      if (upd) {
+        d = group16_max(d);
+        if ((lane & 15) == 0 && i < row1) a.seg_s[i >> 4] = (uint8_t)rule_level(d);
+      }
+    }
+    grid_sync(a.gs, epoch, &lds_i);
+    // t = A s, t.s and t.t
+    double tw[2] = {0.0, 0.0};
+    for (int bi = tr0; bi < tr1; ++bi) {
+      const double tvv = tile_row_sum<MODE, mixed>(a.A, bi, lane, a.s, a.s32, a.seg_s, force64, &cnt);
+      if (lane < 16) {
+        const int i = bi * 16 + lane;
+        t[i] = tvv;
+        tw[0] += tvv * a.s[i];
+        tw[1] += tvv * tvv;
+      }
+    }
+    grid_sum<2>(tw, a.partials, a.gs, epoch, slot, lds, &lds_i);
+    const double omega = tw[1] != 0.0 && isfinite(tw[1]) ? tw[0] / tw[1] : 0.0;
+    // x += alpha p + omega s, r = s - omega t, rho' = r.rh, ||r||^2
+    double nw[2] = {0.0, 0.0};
+    for (int i = row0 + lane; i < row1; i += 64) {
+      a.x[i] += alpha * a.p[i] + omega * a.s[i];
+      const double rv = a.s[i] - omega * t[i];
+      a.r[i] = rv;
+      nw[0] += rv * a.rh[i];
+      nw[1] += rv * rv;
+    }
+    grid_sum<2>(nw, a.partials, a.gs, epoch, slot, lds, &lds_i);
+    double rho_new = nw[0], rr_new = nw[1];
+    if (a.fixed_iters > 0) {
+      if (it >= a.fixed_iters) { rr = rr_new; break; }
+    } else {
+      const bool converged = rr_new <= stop2;
+      const bool periodic = mixed && !force64 &&
+                            (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
+      if (converged || periodic || omega == 0.0 || !isfinite(rr_new)) {
+        double tr[2] = {0.0, 0.0};
+        for (int bi = tr0; bi < tr1; ++bi) {
+          const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
+          if (lane < 16) {
+            const int i = bi * 16 + lane;
+            const double rv = a.b[i] - ax;
+            a.r[i] = rv;
+            tr[0] += rv * rv;
+            tr[1] += rv * a.rh[i];
+          }
+        }
+        grid_sum<2>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
+        ++refresh;
+        it_ref = it;
+        if (tr[0] <= stop2) { rr = tr[0]; break; }
+        if (mixed && !force64 && (tr[0] > 0.25 * rr_ref || refresh > a.max_refresh)) {
+          force64 = 1;
+          restart = true;
+        }
+        rr_new = tr[0];
+        rho_new = tr[1];
+        rr_ref = tr[0];
+      }
+      if (it >= a.maxiter) { rr = rr_new; status = MF_NOT_CONVERGED; break; }
+      if (omega == 0.0 || rho == 0.0 || rho_new == 0.0 || !isfinite(rho_new)) restart = true;
+      if (restart) continue;
+    }
+    const double beta = (rho_new / rho) * (alpha / omega);
+    rho = rho_new;
+    rr = rr_new;
+    // p = r + beta (p - omega v)
+    const bool updp = MODE == kPerSeg && (it % a.check_interval) == 0;
+    for (int base = row0; base < row1; base += 64) {
+      const int i = base + lane;
+      double d = 0.0;
+      if (i < row1) {
+        const double po = a.p[i];
+        const double pv = a.r[i] + beta * (po - omega * v[i]);
+        a.p[i] = pv;
+        if (mixed) a.p32[i] = (float)pv;
+        d = fabs(pv - po);
+      }
+      if (updp) {
+        d = group16_max(d);
+        if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
+      }
+    }
+    grid_sync(a.gs, epoch, &lds_i);
+  }
+  if (mixed) flush_hist(cnt, a.hist, lds);
+  double fin[1] = {true_residual_rows(a, tr0, tr1, lane, nullptr)};
+  grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.stats[kStIters] = it;
+    a.stats[kStRefresh] = refresh;
+    a.stats[kStFallback] = force64;
+    a.stats[kStRecursive] = rr;
+    a.stats[kStTrue] = fin[0];
+    a.stats[kStBB] = bb;
+    a.stats[kStStatus] = status;
+  }
+}
+
+// Error-bound policy: every tile takes the cheapest payload whose largest
+// quantisation error is within `theta` of the tile's largest magnitude
+// (INT8, then FP16, then FP32).  With theta at FP32 rounding (2^-24) a tile
+// drops below FP32 only when INT8 or FP16 represent it as well as FP32 does,
+// e.g. the constant couplings of a stencil.
+__global__ void k_errbound_levels(int T, const double *__restrict__ qerr,
+                                  const double *__restrict__ amax, uint8_t *__restrict__ prec,
+                                  double theta) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const double lim = theta * amax[t];
+  uint8_t l = MF_PREC_FP32;
+  if (qerr[2 * T + t] <= lim) l = MF_PREC_INT8;
+  else if (qerr[T + t] <= lim) l = MF_PREC_FP16;
+  prec[t] = l;
+}
+
+// ------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------
+
+namespace {
+
+struct Workspace {
+  double *r, *p, *q, *rh, *s, *tv;
+  float *p32, *s32;
+  uint8_t *seg_p, *seg_s;
+  double *partials;
+  GridSync *gs;
+  double *stats;
+  unsigned long long *hist;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Carves the handle's device workspace (grown on demand).
+mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
+  const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
+  size_t need = 6 * align256(n * 8) + 2 * align256(n * 4) + 2 * align256(segs) +
+                align256(2 * kMaxGridSum * (size_t)G * 8) + align256(sizeof(GridSync)) + align256(kStCount * 8) +
+                align256(5 * 8);
+  if (t->work_bytes < need) {
+    if (t->work) MF_HIP(hipFree(t->work));
+    t->work = nullptr;
+    t->work_bytes = 0;
+    MF_HIP(hipMalloc(&t->work, need));
+    t->work_bytes = need;
+  }
+  char *c = (char *)t->work;
+  auto take = [&](size_t bytes) { char *p = c; c += align256(bytes); return (void *)p; };
+  w.r = (double *)take(n * 8);
+  w.p = (double *)take(n * 8);
+  w.q = (double *)take(n * 8);
+  w.rh = (double *)take(n * 8);
+  w.s = (double *)take(n * 8);
+  w.tv = (double *)take(n * 8);
+  w.p32 = (float *)take(n * 4);
+  w.s32 = (float *)take(n * 4);
+  w.seg_p = (uint8_t *)take(segs);
+  w.seg_s = (uint8_t *)take(segs);
+  w.partials = (double *)take(2 * kMaxGridSum * (size_t)G * 8);
+  w.gs = (GridSync *)take(sizeof(GridSync));
+  w.stats = (double *)take(kStCount * 8);
+  w.hist = (unsigned long long *)take(5 * 8);
+  return MF_OK;
+}
+
+template <typename K>
+mf_status grid_for(K kernel, int override_blocks, int *G) {
+  int dev = 0, ncu = 0, nb = 0;
+  MF_HIP(hipGetDevice(&dev));
+  MF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, 0));
+  if (nb < 1) return fail(MF_ERR_COOP, "persistent kernel does not fit on a CU");
+  const int maxg = nb * ncu;
+  *G = override_blocks > 0 ? std::min(override_blocks, maxg) : maxg;
+  if (*G > kMaxBlocks) *G = kMaxBlocks;
+  return MF_OK;
+}
+
+mf_status ensure_schedule(mf_tiled *t, int nwaves) {
+  if (t->sched.wave_tr && t->sched_waves_for == nwaves) return MF_OK;
+  if (t->sched.wave_tr) MF_HIP(hipFree(t->sched.wave_tr));
+  t->sched.wave_tr = nullptr;
+  std::vector<int> b = balance_tilerows(t, nwaves);
+  MF_HIP(hipMalloc(&t->sched.wave_tr, sizeof(int) * (nwaves + 1)));
+  MF_HIP(hipMemcpy(t->sched.wave_tr, b.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice));
+  t->sched.nwaves = nwaves;
+  t->sched_waves_for = nwaves;
+  return MF_OK;
+}
+
+int spmv_mode_for(int policy) {
+  switch (policy) {
+    case MF_POLICY_FP64: return kAll64;
+    case MF_POLICY_RULE: return kPerSeg;
+    default: return kPerTile;
+  }
+}
+
+}  // namespace
+
+mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_solve_opts *o,
+                         mf_report *rep, bool bicg) {
+  if (!t || !d_b || !d_x) return fail(MF_ERR_ARG, "solve: null argument");
+  if (t->dev.device < 0) return fail(MF_ERR_ARG, "solve: host-only handle");
+  mf_solve_opts opt;
+  mf_solve_opts_default(&opt);
+  if (o) opt = *o;
+  if (opt.maxiter <= 0) opt.maxiter = 10000;
+  if (opt.check_interval <= 0) opt.check_interval = 8;
+  if (!(opt.refresh_ratio > 0.0) || opt.refresh_ratio >= 1.0) opt.refresh_ratio = 0.1;
+  if (opt.max_refresh <= 0) opt.max_refresh = 64;
+  if (opt.refresh_every <= 0) opt.refresh_every = 64;
+  if (!(opt.tol > 0.0)) opt.tol = 1e-10;
+  if (opt.policy < MF_POLICY_FP64 || opt.policy > MF_POLICY_ERRBOUND)
+    return fail(MF_ERR_ARG, "solve: unknown policy");
+  if (opt.policy == MF_POLICY_LEARNED && !t->dev.has_mlp)
+    return fail(MF_ERR_ARG, "solve: learned policy requested but no MLP installed (mf_policy_set_mlp)");
+  MF_HIP(hipSetDevice(t->dev.device));
+  hipStream_t st = (hipStream_t)opt.stream;
+  const int mode = spmv_mode_for(opt.policy);
+
+  const void *kern;
+  if (bicg)
+    kern = mode == kAll64 ? (const void *)k_bicgstab<kAll64>
+         : mode == kPerSeg ? (const void *)k_bicgstab<kPerSeg> : (const void *)k_bicgstab<kPerTile>;
+  else
+    kern = mode == kAll64 ? (const void *)k_cg<kAll64>
+         : mode == kPerSeg ? (const void *)k_cg<kPerSeg> : (const void *)k_cg<kPerTile>;
+  int G = 0;
+  mf_status sst = grid_for(kern, opt.blocks, &G);
+  if (sst != MF_OK) return sst;
+  sst = ensure_schedule(t, G * kWavesPerBlock);
+  if (sst != MF_OK) return sst;
+  Workspace w;
+  sst = get_workspace(t, G, w);
+  if (sst != MF_OK) return sst;
+
+  const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
+  MF_HIP(hipMemsetAsync(w.gs, 0, sizeof(GridSync), st));
+  MF_HIP(hipMemsetAsync(w.hist, 0, 5 * 8, st));
+  MF_HIP(hipMemsetAsync(w.stats, 0, kStCount * 8, st));
+  MF_HIP(hipMemsetAsync(w.seg_p, MF_PREC_FP64, segs, st));
+  MF_HIP(hipMemsetAsync(w.seg_s, MF_PREC_FP64, segs, st));
+  // Padding entries of the work vectors must be zero: tiles whose columns
+  // lie beyond the rows gather them.
+  MF_HIP(hipMemsetAsync(w.r, 0, 6 * align256(n * 8), st));
+  MF_HIP(hipMemsetAsync(w.p32, 0, 2 * align256(n * 4), st));
+  if (opt.policy == MF_POLICY_FIXED)
+    MF_HIP(hipMemcpyAsync(t->dev.prec, t->dev.prec_user, t->tilenum, hipMemcpyDeviceToDevice, st));
+  if (opt.policy == MF_POLICY_ERRBOUND && t->tilenum > 0) {
+    k_errbound_levels<<<(t->tilenum + 255) / 256, 256, 0, st>>>(t->tilenum, t->dev.qerr, t->dev.amax,
+                                                               t->dev.prec, 0x1p-24);
+    MF_HIP(hipGetLastError());
+  }
+
+  SolveArgs a;
+  a.A = view_of(t);
+  a.b = d_b;
+  a.x = d_x;
+  a.r = w.r; a.p = w.p; a.q = w.q; a.rh = w.rh; a.s = w.s; a.tv = w.tv;
+  a.p32 = w.p32; a.s32 = w.s32;
+  a.seg_p = w.seg_p; a.seg_s = w.seg_s;
+  a.wave_tr = t->sched.wave_tr;
+  a.partials = w.partials;
+  a.gs = w.gs;
+  a.stats = w.stats;
+  a.hist = w.hist;
+  a.tol2 = opt.tol * opt.tol;
+  a.maxiter = opt.maxiter;
+  a.fixed_iters = opt.fixed_iters;
+  a.policy = opt.policy;
+  a.check_interval = opt.check_interval;
+  a.ratio2 = opt.refresh_ratio * opt.refresh_ratio;
+  a.max_refresh = opt.max_refresh;
+  a.refresh_every = opt.refresh_every;
+
+  hipEvent_t e0, e1;
+  MF_HIP(hipEventCreate(&e0));
+  MF_HIP(hipEventCreate(&e1));
+  MF_HIP(hipEventRecord(e0, st));
+  void *args[] = {&a};
+  // The grid never exceeds what the occupancy query says is resident at once.
+  // A cooperative launch also guarantees it; when the runtime refuses one
+  // (e.g. another process holds the device's cooperative queue) the same grid
+  // is launched normally: the barrier needs only atomics, and its bounded
+  // spin turns a block that is not resident into a reported abort.
+  hipError_t le = hipLaunchCooperativeKernel(kern, dim3(G), dim3(kBlock), args, 0, st);
+  if (le == hipErrorIllegalState || le == hipErrorNotSupported) {
+    (void)hipGetLastError();
+    le = hipLaunchKernel(kern, dim3(G), dim3(kBlock), args, 0, st);
+  }
+  if (le != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return fail(MF_ERR_COOP, std::string("hipLaunchCooperativeKernel: ") + hipGetErrorString(le));
+  }
+  MF_HIP(hipEventRecord(e1, st));
+  MF_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  MF_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+
+  double stats[kStCount];
+  unsigned long long hist[5];
+  GridSync gsh;
+  MF_HIP(hipMemcpy(&gsh, w.gs, sizeof(gsh), hipMemcpyDeviceToHost));
+  if (gsh.abort) return fail(MF_ERR_COOP, "solve: grid barrier timed out (grid not co-resident?)");
+  MF_HIP(hipMemcpy(stats, w.stats, sizeof(stats), hipMemcpyDeviceToHost));
+  MF_HIP(hipMemcpy(hist, w.hist, sizeof(hist), hipMemcpyDeviceToHost));
+  const double bb = stats[kStBB];
+  const int status = (int)stats[kStStatus];
+  mf_report r;
+  std::memset(&r, 0, sizeof(r));
+  r.iterations = (int)stats[kStIters];
+  r.refreshes = (int)stats[kStRefresh];
+  r.fell_back = (int)stats[kStFallback];
+  r.rel_residual = bb > 0 ? std::sqrt(stats[kStTrue] / bb) : std::sqrt(stats[kStTrue]);
+  r.rel_residual_recursive = bb > 0 ? std::sqrt(stats[kStRecursive] / bb) : 0.0;
+  r.time_ms = ms;
+  for (int i = 0; i < 5; ++i) r.prec_hist[i] = (long long)hist[i];
+  r.grid_blocks = G;
+  int ret = status;
+  if (ret == MF_OK && opt.fixed_iters <= 0 && !(r.rel_residual <= opt.tol)) ret = MF_NOT_CONVERGED;
+  r.status = ret;
+  if (rep) *rep = r;
+  if (ret == MF_ERR_BREAKDOWN) set_error("solve: breakdown (zero or non-finite denominator)");
+  if (ret == MF_NOT_CONVERGED) set_error("solve: tolerance not reached");
+  return ret;
+}
+
+}  // namespace mf
+
+using namespace mf;
+
+extern "C" void mf_solve_opts_default(mf_solve_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->tol = 1e-10;
+  o->maxiter = 10000;
+  o->policy = MF_POLICY_FP64;
+  o->check_interval = 8;
+  o->refresh_ratio = 0.1;
+  o->max_refresh = 64;
+  o->refresh_every = 64;
+}
+
+extern "C" mf_status mf_cg_solve(mf_tiled *t, const double *d_b, double *d_x,
+                                 const mf_solve_opts *opts, mf_report *rep) {
+  return run_persistent(t, d_b, d_x, opts, rep, false);
+}
+
+extern "C" mf_status mf_bicgstab_solve(mf_tiled *t, const double *d_b, double *d_x,
+                                       const mf_solve_opts *opts, mf_report *rep) {
+  return run_persistent(t, d_b, d_x, opts, rep, true);
+}

@@ -1,0 +1,252 @@
+// mf_tiles.hip -- device layout of the tiled matrix: upload, payload packing
+// (FP32 / FP16 / INT8 copies with per-tile scales, computed on the GPU) and
+// per-tile precision control.
+//
+// HBM layout per handle (T = tilenum, Z = nnz):
+//   tile_ptr  (tilem+1) i32    tile-row -> first tile
+//   tile_col  T i32            tile column
+//   tile_off  (T+1) i32        payload offset of each tile (Z at the end)
+//   tile_row  T i32            tile-row of each tile
+//   rowptr    16T u8           per tile, start offset of each of its 16 rows
+//   col       Z u8             tile-local column
+//   v64/v32/v16/v8             Z entries each: FP64, FP32, FP16 (scaled by
+//                              a power of two per tile), INT8 (max/127 scale)
+//   s16, s8   T f32            per-tile scales
+//   qerr      3T f64           per-tile max |a - dequant(a)| for FP32/16/INT8
+//   amax      T f64            per-tile max |a|
+//   prec      T u8             current precision level of each tile
+// The reference's row pointers are 8-bit row *starts* (src/csr2block.h:252),
+// which never exceed 240, so 8 bits suffice for any tile; row ends come from
+// the next row's start or the tile's nnz.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mf_device.h"
+
+namespace mf {
+
+__global__ void k_pack_payloads(int T, const int *__restrict__ off, const double *__restrict__ v64,
+                                float *__restrict__ v32, uint16_t *__restrict__ v16,
+                                int8_t *__restrict__ v8, float *__restrict__ s16,
+                                float *__restrict__ s8, double *__restrict__ qerr,
+                                double *__restrict__ amax) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const int a = off[t], b = off[t + 1];
+  double m = 0.0;
+  for (int k = a; k < b; ++k) m = fmax(m, fabs(v64[k]));
+  amax[t] = m;
+  // FP16: power-of-two scale 2^e with e = floor(log2 max|a|), so the scaled
+  // magnitudes lie in [.., 2); scaling by a power of two is exact in FP32, so
+  // for in-range tiles the stored value equals (half)(float)a.
+  int e = m > 0.0 ? ilogb(m) : 0;
+  e = e < -100 ? -100 : (e > 100 ? 100 : e);
+  const float sc16 = ldexpf(1.0f, e);
+  const float inv16 = ldexpf(1.0f, -e);
+  // INT8: smallest power-of-two scale with max|a|/scale <= 127 (exact
+  // division; tiles of multiples of the scale, such as the -1 couplings of a
+  // Laplacian, are stored exactly), round to nearest even.
+  float sc8 = 1.0f;
+  if (m > 0.0) {
+    int e8 = ilogb(m / 127.0);
+    if (ldexp(127.0, e8) < m) ++e8;
+    e8 = e8 < -120 ? -120 : (e8 > 120 ? 120 : e8);
+    sc8 = ldexpf(1.0f, e8);
+  }
+  s16[t] = sc16;
+  s8[t] = sc8;
+  double e32 = 0.0, e16 = 0.0, e8 = 0.0;
+  for (int k = a; k < b; ++k) {
+    const double v = v64[k];
+    const float f = (float)v;
+    v32[k] = f;
+    e32 = fmax(e32, fabs(v - (double)f));
+    const _Float16 h = (_Float16)(f * inv16);
+    v16[k] = __builtin_bit_cast(uint16_t, h);
+    e16 = fmax(e16, fabs(v - (double)((float)h * sc16)));
+    double q = rint(v / (double)sc8);
+    q = q > 127.0 ? 127.0 : (q < -127.0 ? -127.0 : q);
+    v8[k] = (int8_t)q;
+    e8 = fmax(e8, fabs(v - q * (double)sc8));
+  }
+  qerr[t] = e32;
+  qerr[T + t] = e16;
+  qerr[2 * T + t] = e8;
+}
+
+template <typename T>
+static mf_status dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(MF_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return MF_OK;
+}
+
+#define MF_TRY(x)                 \
+  do {                            \
+    mf_status s_ = (x);           \
+    if (s_ != MF_OK) return s_;   \
+  } while (0)
+
+mf_status upload_tiles(mf_tiled *t, int device) {
+  DeviceLayout &d = t->dev;
+  MF_HIP(hipSetDevice(device));
+  d.device = device;
+  const int T = t->tilenum, Z = t->csrsize, TM = t->tilem;
+  d.npad = kTile * std::max(t->tilem, t->tilen);
+  if (d.npad == 0) d.npad = kTile;
+
+  std::vector<int> tile_row(T);
+  std::vector<uint8_t> rp((size_t)T * kTile);
+  std::vector<int> diag(TM, -1);
+  for (int bi = 0; bi < TM; ++bi) {
+    const int h = bi == TM - 1 ? t->rowA - (TM - 1) * kTile : kTile;
+    for (int tt = t->tile_ptr[bi]; tt < t->tile_ptr[bi + 1]; ++tt) {
+      tile_row[tt] = bi;
+      if (t->tile_columnidx[tt] == bi) diag[bi] = tt;
+      const int nzt = t->csr_offset[tt + 1] - t->csr_offset[tt];
+      for (int r = 0; r < kTile; ++r)
+        rp[(size_t)tt * kTile + r] = r < h ? t->Blockcsr_Ptr[t->csrptr_offset[tt] + r] : (uint8_t)std::min(nzt, 255);
+    }
+  }
+  // A 256-entry tile (16 rows x 16 columns) has row 15 starting at 240 and
+  // ends at nnz = 256, which the SpMV reads from tile_off; only padded rows
+  // of a short last tile-row store the tile nnz here, and those tiles have at
+  // most h*16 < 256 entries.
+  MF_TRY(dalloc(&d.tile_ptr, TM + 1));
+  MF_TRY(dalloc(&d.tile_col, T));
+  MF_TRY(dalloc(&d.tile_off, T + 1));
+  MF_TRY(dalloc(&d.tile_row, T));
+  MF_TRY(dalloc(&d.rowptr, (size_t)T * kTile));
+  MF_TRY(dalloc(&d.col, Z));
+  MF_TRY(dalloc(&d.v64, Z));
+  MF_TRY(dalloc(&d.v32, Z));
+  MF_TRY(dalloc(&d.v16, Z));
+  MF_TRY(dalloc(&d.v8, Z));
+  MF_TRY(dalloc(&d.s16, T));
+  MF_TRY(dalloc(&d.s8, T));
+  MF_TRY(dalloc(&d.qerr, 3 * (size_t)T));
+  MF_TRY(dalloc(&d.amax, T));
+  MF_TRY(dalloc(&d.prec, T));
+  MF_TRY(dalloc(&d.prec_user, T));
+  MF_TRY(dalloc(&d.diag_tile, TM));
+  MF_TRY(dalloc(&d.mlp16, MF_MLP_NPARAM));
+  MF_HIP(hipMemcpy(d.tile_ptr, t->tile_ptr.data(), sizeof(int) * (TM + 1), hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.tile_col, t->tile_columnidx.data(), sizeof(int) * T, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.tile_off, t->csr_offset.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.tile_row, tile_row.data(), sizeof(int) * T, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.rowptr, rp.data(), (size_t)T * kTile, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.col, t->Tile_csr_Col.data(), Z, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.v64, t->Blockcsr_Val.data(), sizeof(double) * Z, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.diag_tile, diag.data(), sizeof(int) * TM, hipMemcpyHostToDevice));
+  MF_HIP(hipMemset(d.prec, MF_PREC_FP64, T));
+  MF_HIP(hipMemset(d.prec_user, MF_PREC_FP64, T));
+  if (T > 0) {
+    k_pack_payloads<<<(T + 255) / 256, 256>>>(T, d.tile_off, d.v64, d.v32, d.v16, d.v8, d.s16,
+                                              d.s8, d.qerr, d.amax);
+    MF_HIP(hipGetLastError());
+  }
+  // The FP32 payload keeps the caller's FP32 copy when one was given
+  // (the reference passes Val_Low, src/csr2block.h:248).
+  MF_HIP(hipMemcpy(d.v32, t->Blockcsr_Val_Low.data(), sizeof(float) * Z, hipMemcpyHostToDevice));
+  MF_HIP(hipDeviceSynchronize());
+  return MF_OK;
+}
+
+void free_device(mf_tiled *t) {
+  DeviceLayout &d = t->dev;
+  if (d.device >= 0) (void)hipSetDevice(d.device);
+  void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rowptr, d.col,
+                  d.v64,      d.v32,      d.v16,      d.v8,       d.s16,    d.s8,
+                  d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp16,
+                  t->sched.wave_tr, t->work};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  d = DeviceLayout();
+  t->sched = Schedule();
+  t->work = nullptr;
+  t->work_bytes = 0;
+}
+
+}  // namespace mf
+
+using namespace mf;
+
+extern "C" mf_status mf_tile_create(int rowA, int colA, const int *rowptr, const int *colidx,
+                                    const double *val, const float *val_low, int device,
+                                    mf_tiled **out) {
+  if (!out) return fail(MF_ERR_ARG, "mf_tile_create: null output");
+  *out = nullptr;
+  if (device != -1) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      return fail(MF_ERR_HIP, "mf_tile_create: no HIP device");
+    if (device < 0 || device >= ndev) return fail(MF_ERR_ARG, "mf_tile_create: bad device");
+  }
+  mf_tiled *t = new mf_tiled();
+  mf_status st = build_tiles_host(t, rowA, colA, rowptr, colidx, val, val_low);
+  if (st == MF_OK && device != -1) st = upload_tiles(t, device);
+  if (device == -1) t->dev.npad = kTile * std::max(std::max(t->tilem, t->tilen), 1);
+  if (st != MF_OK) {
+    std::string msg = mf_last_error();
+    free_device(t);
+    delete t;
+    set_error(msg);
+    return st;
+  }
+  *out = t;
+  return MF_OK;
+}
+
+extern "C" void mf_tile_free(mf_tiled *t) {
+  if (!t) return;
+  free_device(t);
+  delete t;
+}
+
+extern "C" mf_status mf_tile_set_precision(mf_tiled *t, const unsigned char *levels, int uniform) {
+  if (!t) return fail(MF_ERR_ARG, "mf_tile_set_precision: null handle");
+  if (t->dev.device < 0) return fail(MF_ERR_ARG, "mf_tile_set_precision: host-only handle");
+  MF_HIP(hipSetDevice(t->dev.device));
+  const int T = t->tilenum;
+  if (levels) {
+    for (int i = 0; i < T; ++i)
+      if (levels[i] > MF_PREC_SKIP) return fail(MF_ERR_ARG, "mf_tile_set_precision: bad level");
+    MF_HIP(hipMemcpy(t->dev.prec, levels, T, hipMemcpyHostToDevice));
+    MF_HIP(hipMemcpy(t->dev.prec_user, levels, T, hipMemcpyHostToDevice));
+  } else {
+    if (uniform < 0 || uniform > MF_PREC_SKIP) return fail(MF_ERR_ARG, "mf_tile_set_precision: bad level");
+    MF_HIP(hipMemset(t->dev.prec, uniform, T));
+    MF_HIP(hipMemset(t->dev.prec_user, uniform, T));
+  }
+  return MF_OK;
+}
+
+extern "C" mf_status mf_tile_get_precision(const mf_tiled *t, unsigned char *levels) {
+  if (!t || !levels) return fail(MF_ERR_ARG, "mf_tile_get_precision: null argument");
+  if (t->dev.device < 0) return fail(MF_ERR_ARG, "mf_tile_get_precision: host-only handle");
+  MF_HIP(hipSetDevice(t->dev.device));
+  MF_HIP(hipMemcpy(levels, t->dev.prec, t->tilenum, hipMemcpyDeviceToHost));
+  return MF_OK;
+}
+
+extern "C" mf_status mf_tile_get_scales(const mf_tiled *t, float *scale16, float *scale8) {
+  if (!t) return fail(MF_ERR_ARG, "mf_tile_get_scales: null handle");
+  if (t->dev.device < 0) return fail(MF_ERR_ARG, "mf_tile_get_scales: host-only handle");
+  MF_HIP(hipSetDevice(t->dev.device));
+  if (scale16) MF_HIP(hipMemcpy(scale16, t->dev.s16, sizeof(float) * t->tilenum, hipMemcpyDeviceToHost));
+  if (scale8) MF_HIP(hipMemcpy(scale8, t->dev.s8, sizeof(float) * t->tilenum, hipMemcpyDeviceToHost));
+  return MF_OK;
+}
+
+extern "C" mf_status mf_tile_get_quant_error(const mf_tiled *t, double *err) {
+  if (!t || !err) return fail(MF_ERR_ARG, "mf_tile_get_quant_error: null argument");
+  if (t->dev.device < 0) return fail(MF_ERR_ARG, "mf_tile_get_quant_error: host-only handle");
+  MF_HIP(hipSetDevice(t->dev.device));
+  MF_HIP(hipMemcpy(err, t->dev.qerr, sizeof(double) * 3 * (size_t)t->tilenum, hipMemcpyDeviceToHost));
+  return MF_OK;
+}

@@ -1,0 +1,191 @@
+"""Product host code (reader, tile build, reference-derived arrays) against the
+golden fixtures and the oracle, through the C ABI with host-only handles
+(device = -1: no HIP calls).  CPU only."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+from test_oracle import FIXTURES, MTX, TILE_KEYS, tiles_from_golden
+
+
+def host_handle(mf, g):
+    return mf.TiledMatrix(g["csr_rowptr"], g["csr_colidx"], g["csr_val"], rowA=int(g["meta_rowA"]),
+                          colA=int(g["meta_colA"]), device=-1)
+
+
+@pytest.mark.parametrize("name", [k for k in FIXTURES if k in MTX])
+def test_product_reader_matches_reference(mf, golden, name):
+    g = golden(name)
+    m, n, rp, ci, va, sym = mf.read_mtx(os.path.join(GOLDEN, MTX[name]))
+    assert (m, n, sym) == (int(g["meta_m"]), int(g["meta_n"]), int(g["meta_is_sym"]))
+    np.testing.assert_array_equal(rp, g["csr_rowptr"])
+    np.testing.assert_array_equal(ci, g["csr_colidx"])
+    np.testing.assert_array_equal(va, g["csr_val"])
+
+
+def test_product_reader_rejects_bad_input(mf, tmp_path):
+    for text in ["junk\n", "%%MatrixMarket matrix array real general\n2 2\n",
+                 "%%MatrixMarket matrix coordinate real general\n2 2 1\n3 1 1\n"]:
+        p = tmp_path / "bad.mtx"
+        p.write_text(text)
+        with pytest.raises(mf.MfError):
+            mf.read_mtx(p)
+    with pytest.raises(mf.MfError):
+        mf.read_mtx(tmp_path / "missing.mtx")
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_product_tiles_match_reference(mf, golden, name):
+    g = golden(name)
+    t = host_handle(mf, g)
+    assert (t.tilem, t.tilen, t.tilenum, t.csrsize, t.csrptrlen) == tuple(
+        int(g["meta_" + k]) for k in ("tilem", "tilen", "tilenum", "csrsize", "csrptrlen"))
+    a = t.arrays()
+    for k in TILE_KEYS:
+        np.testing.assert_array_equal(a[k], g[k], err_msg=k)
+    o = t.offsets()
+    np.testing.assert_array_equal(o["ptroffset1"], g["ptroffset1"])
+    np.testing.assert_array_equal(o["ptroffset2"], g["ptroffset2"])
+    np.testing.assert_array_equal(o["blkcoostylerowidx"], g["blkcoostylerowidx"])
+    np.testing.assert_array_equal(o["colstart"], g["blkcoostylerowidx_colstart"])
+    np.testing.assert_array_equal(o["colstop"], g["blkcoostylerowidx_colstop"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_product_compaction_and_balance_match_oracle(mf, oracle, golden, name):
+    g = golden(name)
+    t = host_handle(mf, g)
+    ot = tiles_from_golden(oracle, g)
+    c, oc = t.compact(), oracle.compact(ot)
+    for k in oc:
+        np.testing.assert_array_equal(c[k], oc[k], err_msg=k)
+    if ot["tilenum"] > 1:
+        b = t.balance(6912)
+        ob = oracle.load_balance(ot, int(g["meta_nnz"]))
+        assert b["nwarps"] == ob["nwarps"]
+        np.testing.assert_array_equal(b["balance_tile_ptr"], ob["balance_tile_ptr"])
+        np.testing.assert_array_equal(b["index_each_block"], ob["index_each_block"])
+        np.testing.assert_array_equal(b["row_each_block"], ob["row_each_block"])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_product_tiles_random_matrices(mf, oracle, seed):
+    """Random sparsity (rows with many tiles, empty rows, a short last
+    tile-row, dense tiles) against the oracle restatement."""
+    rng = np.random.default_rng(seed)
+    m, n = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+    dens = rng.uniform(0.001, 0.3)
+    mask = rng.random((m, n)) < dens
+    if seed == 3:
+        mask[:16, :16] = True  # one dense 256-entry tile
+    rows, cols = np.nonzero(mask)
+    # shuffle the order inside each row (file order is kept, not sorted)
+    perm = np.lexsort((rng.random(len(rows)), rows))
+    rows, cols = rows[perm], cols[perm]
+    vals = rng.standard_normal(len(rows))
+    rp = np.zeros(m + 1, np.int32)
+    np.cumsum(np.bincount(rows, minlength=m), out=rp[1:])
+    rowA = m if seed % 2 else (m // 16) * 16
+    t = mf.TiledMatrix(rp, cols.astype(np.int32), vals, rowA=rowA, colA=n, device=-1)
+    ot = oracle.tile_create(rowA, n, rp, cols, vals)
+    a = t.arrays()
+    for k in TILE_KEYS:
+        np.testing.assert_array_equal(a[k], ot[k], err_msg=k)
+
+
+def test_bad_column_is_rejected(mf):
+    with pytest.raises(mf.MfError):
+        mf.TiledMatrix(np.array([0, 1], np.int32), np.array([5], np.int32), np.ones(1), colA=4, device=-1)
+
+
+def test_cbd_round_trip(mf, tmp_path):
+    import ctypes as C
+    from millefeuille._lib import MfCsr, lib
+    m, n, rp, ci, va, _ = mf.read_mtx(os.path.join(GOLDEN, "add20.mtx"))
+    a = MfCsr(m, n, len(va), rp.ctypes.data_as(C.POINTER(C.c_int)), ci.ctypes.data_as(C.POINTER(C.c_int)),
+              va.ctypes.data_as(C.POINTER(C.c_double)), 0)
+    p = str(tmp_path / "add20.cbd").encode()
+    assert lib.mf_write_cbd(p, C.byref(a)) == 0
+    b = MfCsr()
+    assert lib.mf_read_cbd(p, C.byref(b)) == 0
+    try:
+        assert (b.m, b.n, b.nnz) == (m, n, len(va))
+        np.testing.assert_array_equal(np.ctypeslib.as_array(b.rowptr, (m + 1,)), rp)
+        np.testing.assert_array_equal(np.ctypeslib.as_array(b.colidx, (len(va),)), ci)
+        np.testing.assert_array_equal(np.ctypeslib.as_array(b.val, (len(va),)), va)
+    finally:
+        lib.mf_csr_free(C.byref(b))
+    # the layout is the reference's: int32 rows, int32 cols, int64 nnz, int64 rowptr, ...
+    raw = (tmp_path / "add20.cbd").read_bytes()
+    assert np.frombuffer(raw[:8], np.int32).tolist() == [m, n]
+    assert np.frombuffer(raw[8:16], np.int64)[0] == len(va)
+    assert len(raw) == 16 + 8 * (m + 1) + 4 * len(va) + 8 * len(va)
+
+
+def test_nibble_codec_matches_oracle(oracle):
+    import ctypes as C
+    from millefeuille._lib import lib
+    for n in (0, 1, 2, 7, 64, 255):
+        cols = np.random.default_rng(n).integers(0, 16, n).astype(np.uint8)
+        packed = np.zeros((n + 1) // 2 + 1, np.uint8)
+        lib.mf_encode_nibbles(cols.ctypes.data, packed.ctypes.data, n, 0)
+        np.testing.assert_array_equal(packed[: (n + 1) // 2], oracle.encode(cols))
+        back = np.zeros(max(n, 1), np.uint8)
+        lib.mf_decode_nibbles(packed.ctypes.data, back.ctypes.data, n)
+        np.testing.assert_array_equal(back[:n], cols)
+
+
+def test_read_dmatrix_32_compat(mf, golden):
+    import ctypes as C
+    from millefeuille._lib import lib
+    g = golden("add20")
+    m, n, nnz, sym = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    rp, ci, va = C.POINTER(C.c_int)(), C.POINTER(C.c_int)(), C.POINTER(C.c_double)()
+    lib.read_Dmatrix_32(C.byref(m), C.byref(n), C.byref(nnz), C.byref(rp), C.byref(ci), C.byref(va),
+                        C.byref(sym), os.path.join(GOLDEN, "add20.mtx").encode())
+    assert (m.value, n.value, nnz.value) == (2395, 2395, 17319)
+    np.testing.assert_array_equal(np.ctypeslib.as_array(rp, (m.value + 1,)), g["csr_rowptr"])
+    np.testing.assert_array_equal(np.ctypeslib.as_array(ci, (nnz.value,)), g["csr_colidx"])
+
+
+def test_generators_match_workload_sizes():
+    from millefeuille import gen
+    rp, ci, va = gen.laplace2d(64)
+    assert len(rp) - 1 == 4096 and rp[-1] == 20224
+    rp, ci, va = gen.convdiff3d(20)
+    n = 20 ** 3
+    assert rp[-1] == 7 * n - 6 * 20 * 20
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    assert np.all(np.diff(ci)[np.diff(rows) == 0] > 0)  # ascending columns per row
+    d = va[rows == ci]
+    np.testing.assert_allclose(d, 6 + 2 * 0.5 * 1.75)
+
+
+def test_abi_exports_every_declared_symbol():
+    """The library loads and exports every function include/millefeuille.h
+    declares, and the ctypes table covers all of them."""
+    import ctypes as C
+    from millefeuille._lib import LIB_PATH, SIGNATURES
+    hdr = open(os.path.join(ROOT, "include", "millefeuille.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", hdr))
+    names -= {"if", "sizeof", "defined"}
+    decl = {nm for nm in names if re.search(r"(mf_status|void|int|const char \*)\s*\*?\s*" + nm + r"\s*\(", hdr)}
+    assert len(decl) >= 40
+    lib = C.CDLL(LIB_PATH)
+    for nm in sorted(decl):
+        assert hasattr(lib, nm), nm
+    assert decl <= set(SIGNATURES), sorted(decl - set(SIGNATURES))
+
+
+def test_product_does_not_reference_the_oracle():
+    """The shipped library and package never load or call the oracle."""
+    pkg = os.path.join(ROOT, "mille-feuille_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h", "Makefile")):
+                text = open(os.path.join(dirpath, f), errors="ignore").read()
+                assert "mforacle" not in text and "import oracle" not in text, f
