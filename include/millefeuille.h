@@ -167,6 +167,19 @@ mf_status mf_tile_get_quant_error(const mf_tiled *t, double *err);
 mf_status mf_spmv(mf_tiled *t, const double *d_x, const float *d_x32,
                   double *d_y, int mode, void *stream);
 
+/* Schedule knob for the autotuner: entries per wave of the stand-alone SpMV
+ * (its nnz-balanced tile-row ranges are rebuilt; 0 keeps the current value).
+ * No reference counterpart (the reference hardcodes its chunking,
+ * src/blockspmv_cpu.h:21-31, and its warp budget, src/main-cg.cu:634). */
+mf_status mf_tile_set_schedule(mf_tiled *t, int spmv_wave_nnz);
+
+/* dst[i] = src[i] for n doubles with a plain streaming kernel.  Bandwidth
+ * calibration for the SpMV roofline (the HBM byte counters of a copy of known
+ * size give the scale factor for the SpMV's counters).  No reference
+ * counterpart. */
+mf_status mf_stream_copy(const double *d_src, double *d_dst, int64_t n,
+                         void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* Solvers (single persistent kernel per solve)                            */
 /* ---------------------------------------------------------------------- */

@@ -1,6 +1,6 @@
 // mf_device.h -- device-side building blocks shared by the SpMV and solver
-// kernels: the wave64 tile-row SpMV body, deterministic block / grid
-// reductions and the epoch grid barrier.
+// kernels: the streamed wave64 tiled SpMV, deterministic block / grid
+// reductions and the grid barrier.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,11 +22,11 @@ constexpr int kWave = 64;
 
 // Pointers of the device layout passed by value into kernels.
 struct TileView {
-  const int *tile_ptr;
-  const int *tile_col;
-  const int *tile_off;
-  const uint8_t *rowptr;
-  const uint8_t *col;
+  const int *tile_ptr;  // tilem+1
+  const int *tile_col;  // tilenum
+  const int *tile_off;  // tilenum+1
+  const int *tile_row;  // tilenum
+  const uint8_t *rc;    // nnz: (row << 4) | col inside the tile
   const double *v64;
   const float *v32;
   const uint16_t *v16;
@@ -41,8 +41,8 @@ struct TileView {
 
 inline TileView view_of(const mf_tiled *t) {
   const DeviceLayout &d = t->dev;
-  return TileView{d.tile_ptr, d.tile_col, d.tile_off, d.rowptr, d.col, d.v64,
-                  d.v32,      d.v16,      d.v8,       d.s16,    d.s8,  d.prec,
+  return TileView{d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc,     d.v64,
+                  d.v32,      d.v16,      d.v8,       d.s16,      d.s8,     d.prec,
                   t->tilem,   t->tilenum, d.npad};
 }
 
@@ -55,8 +55,7 @@ __device__ __forceinline__ float half_bits_to_float(uint16_t h) {
 // of x that the tile multiplies, src/main-cg-mixed.cu:236, 1134).
 enum SpmvMode { kAll64 = 0, kPerTile = 1, kPerSeg = 2 };
 
-// Per-lane counters of tile visits per precision level (lane r == 0 of each
-// group counts its tile).
+// Per-lane counters of tile visits per precision level.
 struct PrecCount {
   unsigned c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0;
   __device__ __forceinline__ void add(int p) {
@@ -68,80 +67,293 @@ struct PrecCount {
   }
 };
 
-// One wave computes the 16 row sums of tile-row `bi`.  Lanes are split into
-// four groups of 16 (g = lane>>4); group g walks tiles t0+g, t0+g+4, ...,
-// lane r of the group owns row r of the tile.  Each lane keeps one running
-// FP64 sum across its tiles, and the four groups are folded with two xor
-// shuffles in a fixed order, so the result is deterministic.  On return every
-// lane holds the sum of row (lane & 15).
+// ---------------------------------------------------------------------------
+// Streamed tiled SpMV.
 //
-// Mixed precision: FP64 tiles multiply x (FP64); FP32 / FP16 / INT8 tiles
-// multiply the FP32 copy x32 and accumulate the tile in FP32, then the
-// per-tile scale is applied once per tile row; SKIP tiles are not read.
+// A wave owns a contiguous range of tile-rows, i.e. a contiguous range of
+// tiles and of payload entries (tiles are stored tile-row by tile-row).  It
+// walks that range in windows of at most kWinTiles tiles spanning at most
+// kWinRows tile-rows.  Per window the tile metadata (payload offset, column,
+// tile-row, precision, scale) is staged in wave-private LDS, then the window's
+// entries are streamed: lane L takes the four consecutive entries
+// base+4L..base+4L+3, so one wave instruction covers 256 entries and the
+// values / (row,col) bytes arrive as 32-, 16- or 4-byte vector loads.  Each
+// lane finds the tile of its first entry by a binary search over the staged
+// offsets (overlapping the HBM loads), gathers x, and adds its products into
+// per-row FP64 accumulators in LDS, merging consecutive entries of one row in
+// registers first.  When a window ends, every tile-row it completed is
+// handed to the epilogue row by row (one lane per row); an unfinished last
+// tile-row keeps its accumulators for the next window.
+//
+// Mixed precision: FP64 tiles multiply x; FP32 / FP16 / INT8 tiles multiply
+// the FP32 copy x32 with FP32 products (FP16/INT8 scaled by the tile's power
+// of two) that are accumulated in FP64; SKIP tiles contribute nothing.
 // force64 != 0 treats every tile as FP64 (the solver's fallback).
-template <int MODE, bool COUNT = false>
-__device__ __forceinline__ double tile_row_sum(const TileView &A, int bi, int lane,
-                                               const double *__restrict__ x,
-                                               const float *__restrict__ x32,
-                                               const uint8_t *__restrict__ seglev = nullptr,
-                                               int force64 = 0,
-                                               PrecCount *cnt = nullptr) {
-  const int g = lane >> 4, r = lane & 15;
-  const int t0 = A.tile_ptr[bi], t1 = A.tile_ptr[bi + 1];
-  double acc = 0.0;
-  for (int t = t0 + g; t < t1; t += 4) {
-    const int off = A.tile_off[t];
-    const int nzt = A.tile_off[t + 1] - off;
-    const int tc = A.tile_col[t];
-    const int xo = tc << 4;
-    const int s = A.rowptr[(size_t)t * 16 + r];
-    int e = __shfl_down(s, 1, 16);
-    if (r == 15) e = nzt;
-    int p = MF_PREC_FP64;
-    if (MODE == kPerTile) p = force64 ? MF_PREC_FP64 : A.prec[t];
-    if (MODE == kPerSeg) p = force64 ? MF_PREC_FP64 : seglev[tc];
-    if (COUNT && r == 0) cnt->add(p);
-    if (p == MF_PREC_FP64) {
-      const double *__restrict__ v = A.v64 + off;
-      const uint8_t *__restrict__ c = A.col + off;
-      double a = 0.0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (s + j < e) a += v[s + j] * x[xo + c[s + j]];
-      for (int k = s + 4; k < e; ++k) a += v[k] * x[xo + c[k]];
-      acc += a;
-    } else if (p == MF_PREC_FP32) {
-      const float *__restrict__ v = A.v32 + off;
-      const uint8_t *__restrict__ c = A.col + off;
-      float a = 0.0f;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (s + j < e) a += v[s + j] * x32[xo + c[s + j]];
-      for (int k = s + 4; k < e; ++k) a += v[k] * x32[xo + c[k]];
-      acc += (double)a;
-    } else if (p == MF_PREC_FP16) {
-      const uint16_t *__restrict__ v = A.v16 + off;
-      const uint8_t *__restrict__ c = A.col + off;
-      float a = 0.0f;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (s + j < e) a += half_bits_to_float(v[s + j]) * x32[xo + c[s + j]];
-      for (int k = s + 4; k < e; ++k) a += half_bits_to_float(v[k]) * x32[xo + c[k]];
-      acc += (double)(a * A.s16[t]);
-    } else if (p == MF_PREC_INT8) {
-      const int8_t *__restrict__ v = A.v8 + off;
-      const uint8_t *__restrict__ c = A.col + off;
-      float a = 0.0f;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (s + j < e) a += (float)v[s + j] * x32[xo + c[s + j]];
-      for (int k = s + 4; k < e; ++k) a += (float)v[k] * x32[xo + c[k]];
-      acc += (double)(a * A.s8[t]);
+// ---------------------------------------------------------------------------
+constexpr int kWinTiles = 128;
+constexpr int kWinRows = 16;
+
+struct WaveLds {
+  int toff[kWinTiles + 1];  // payload offset of each tile, relative to the window
+  int tcol[kWinTiles];      // 16 * tile column
+  float tscale[kWinTiles];  // FP16 / INT8 scale of the tile's current level
+  uint8_t trow[kWinTiles];  // tile-row relative to the window's first tile-row
+  uint8_t tprec[kWinTiles];
+  double acc[kWinRows * 16];
+};
+
+template <int MODE>
+__device__ __forceinline__ int tile_level(const TileView &A, const uint8_t *seglev, int t,
+                                          int force64) {
+  if (MODE == kAll64 || force64) return MF_PREC_FP64;
+  if (MODE == kPerTile) return A.prec[t];
+  return seglev[A.tile_col[t]];
+}
+
+// Product of one entry at its tile's level (FP64 result).
+template <int MODE>
+__device__ __forceinline__ double entry_product(const TileView &A, int lvl, int k,
+                                                int xi, const double *__restrict__ x,
+                                                const float *__restrict__ x32, float scale) {
+  if (lvl == MF_PREC_FP64) return A.v64[k] * x[xi];
+  if (lvl == MF_PREC_FP32) return (double)(A.v32[k] * x32[xi]);
+  if (lvl == MF_PREC_FP16) return (double)(half_bits_to_float(A.v16[k]) * x32[xi] * scale);
+  if (lvl == MF_PREC_INT8) return (double)((float)A.v8[k] * x32[xi] * scale);
+  return 0.0;
+}
+
+// One lane's share of a slice: four consecutive entries k0..k0+3, their
+// (row,col) bytes and the raw payload words at the level of the tile that
+// holds the first valid entry (FP64: 8 words, FP32: 4, FP16: 2, INT8: 1).
+struct SliceBuf {
+  int k0;
+  int j;    // window-relative tile of the first valid entry, -1: none valid
+  int lvl;  // level of tile j (the level the payload words were loaded at)
+  uint32_t rc4;
+  uint4 a, b;  // raw payload words 0-3 and 4-7
+};
+
+// Word i of the payload (i is a compile-time constant after unrolling).
+__device__ __forceinline__ uint32_t slice_word(const SliceBuf &s, int i) {
+  switch (i) {
+    case 0: return s.a.x;
+    case 1: return s.a.y;
+    case 2: return s.a.z;
+    case 3: return s.a.w;
+    case 4: return s.b.x;
+    case 5: return s.b.y;
+    case 6: return s.b.z;
+    default: return s.b.w;
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void slice_load(const TileView &A, const WaveLds &L, int base,
+                                           int lane, int Z0, int Z1, int nt, bool all64,
+                                           SliceBuf &s) {
+  s.k0 = base + 4 * lane;
+  s.j = -1;
+  s.lvl = MF_PREC_FP64;
+  if (s.k0 + 3 < Z0 || s.k0 >= Z1) return;
+  s.rc4 = *(const uint32_t *)(A.rc + s.k0);
+  if (all64) {
+    const uint4 *q = (const uint4 *)(A.v64 + s.k0);
+    s.a = q[0];
+    s.b = q[1];
+  }
+  // tile of the first valid entry: binary search over the staged offsets
+  const int kf = (s.k0 > Z0 ? s.k0 : Z0) - Z0;
+  int lo = 0, hi = nt - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L.toff[mid] <= kf) lo = mid;
+    else hi = mid - 1;
+  }
+  s.j = lo;
+  if (!all64) {
+    s.lvl = L.tprec[lo];
+    if (s.lvl == MF_PREC_FP64) {
+      const uint4 *q = (const uint4 *)(A.v64 + s.k0);
+      s.a = q[0];
+      s.b = q[1];
+    } else if (s.lvl == MF_PREC_FP32) {
+      s.a = *(const uint4 *)(A.v32 + s.k0);
+    } else if (s.lvl == MF_PREC_FP16) {
+      const uint2 h = *(const uint2 *)(A.v16 + s.k0);
+      s.a.x = h.x;
+      s.a.y = h.y;
+    } else if (s.lvl == MF_PREC_INT8) {
+      s.a.x = *(const uint32_t *)(A.v8 + s.k0);
     }
   }
-  acc += __shfl_xor(acc, 16);
-  acc += __shfl_xor(acc, 32);
-  return acc;
+}
+
+template <int MODE>
+__device__ __forceinline__ void slice_consume(const TileView &A, WaveLds &L, const SliceBuf &s,
+                                              int Z0, int Z1, bool all64,
+                                              const double *__restrict__ x,
+                                              const float *__restrict__ x32) {
+  if (s.j < 0) return;
+  int j = s.j;
+  int key = -1;
+  double run = 0.0;
+  // Fast path: all four entries valid and inside tile j (the common case:
+  // tiles hold 12-18 entries on the stencil workloads).
+  if (s.k0 >= Z0 && s.k0 + 3 < Z1 && L.toff[j + 1] > s.k0 + 3 - Z0) {
+    const int xo = L.tcol[j];
+    const int ro = L.trow[j] * 16;
+    double prod[4];
+    const int lvl = all64 ? MF_PREC_FP64 : s.lvl;
+    if (lvl == MF_PREC_FP64) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        prod[u] = __builtin_bit_cast(double, ((uint64_t)slice_word(s, 2 * u + 1) << 32) |
+                                                 slice_word(s, 2 * u)) *
+                  x[xo + ((s.rc4 >> (8 * u)) & 15)];
+    } else if (lvl == MF_PREC_FP32) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        prod[u] = (double)(__builtin_bit_cast(float, slice_word(s, u)) *
+                           x32[xo + ((s.rc4 >> (8 * u)) & 15)]);
+    } else if (lvl == MF_PREC_FP16) {
+      const float sc = L.tscale[j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint16_t h = (uint16_t)(slice_word(s, u >> 1) >> (16 * (u & 1)));
+        prod[u] = (double)(half_bits_to_float(h) * x32[xo + ((s.rc4 >> (8 * u)) & 15)] * sc);
+      }
+    } else if (lvl == MF_PREC_INT8) {
+      const float sc = L.tscale[j];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int8_t q = (int8_t)(s.a.x >> (8 * u));
+        prod[u] = (double)((float)q * x32[xo + ((s.rc4 >> (8 * u)) & 15)] * sc);
+      }
+    } else {
+      return;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kk = ro + ((s.rc4 >> (8 * u + 4)) & 15);
+      if (kk != key) {
+        if (key >= 0) atomicAdd(&L.acc[key], run);
+        key = kk;
+        run = prod[u];
+      } else {
+        run += prod[u];
+      }
+    }
+    atomicAdd(&L.acc[key], run);
+    return;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int k = s.k0 + u;
+    if (k < Z0 || k >= Z1) continue;
+    while (L.toff[j + 1] <= k - Z0) ++j;
+    const int rc = (s.rc4 >> (8 * u)) & 0xff;
+    const int xi = L.tcol[j] + (rc & 15);
+    const int kk = L.trow[j] * 16 + (rc >> 4);
+    double prod;
+    if (all64) {
+      prod = __builtin_bit_cast(double, ((uint64_t)slice_word(s, 2 * u + 1) << 32) | slice_word(s, 2 * u)) * x[xi];
+    } else {
+      const int lvl = L.tprec[j];
+      if (lvl != s.lvl) {
+        prod = entry_product<MODE>(A, lvl, k, xi, x, x32, L.tscale[j]);
+      } else if (lvl == MF_PREC_FP64) {
+        prod = __builtin_bit_cast(double, ((uint64_t)slice_word(s, 2 * u + 1) << 32) | slice_word(s, 2 * u)) * x[xi];
+      } else if (lvl == MF_PREC_FP32) {
+        prod = (double)(__builtin_bit_cast(float, slice_word(s, u)) * x32[xi]);
+      } else if (lvl == MF_PREC_FP16) {
+        const uint16_t h = (uint16_t)(slice_word(s, u >> 1) >> (16 * (u & 1)));
+        prod = (double)(half_bits_to_float(h) * x32[xi] * L.tscale[j]);
+      } else if (lvl == MF_PREC_INT8) {
+        const int8_t q = (int8_t)(s.a.x >> (8 * u));
+        prod = (double)((float)q * x32[xi] * L.tscale[j]);
+      } else {
+        prod = 0.0;
+      }
+    }
+    if (kk != key) {
+      if (key >= 0) atomicAdd(&L.acc[key], run);
+      key = kk;
+      run = prod;
+    } else {
+      run += prod;
+    }
+  }
+  if (key >= 0) atomicAdd(&L.acc[key], run);
+}
+
+template <int MODE, bool COUNT, typename Epi>
+__device__ __forceinline__ void wave_spmv(const TileView &A, int tr0, int tr1,
+                                          const double *__restrict__ x,
+                                          const float *__restrict__ x32,
+                                          const uint8_t *__restrict__ seglev, int force64,
+                                          WaveLds &L, int lane, PrecCount *cnt, Epi epi) {
+  if (tr0 >= tr1) return;
+  const bool all64 = MODE == kAll64 || force64;
+  for (int i = lane; i < kWinRows * 16; i += 64) L.acc[i] = 0.0;
+  const int T1 = A.tile_ptr[tr1];
+  int ta = A.tile_ptr[tr0];
+  int ra = tr0;
+  while (ra < tr1) {
+    const int rlim = ra + kWinRows < tr1 ? ra + kWinRows : tr1;
+    int tb = ta + kWinTiles < T1 ? ta + kWinTiles : T1;
+    if (rlim < tr1) {
+      const int tl = A.tile_ptr[rlim];
+      if (tl < tb) tb = tl;
+    }
+    const int nt = tb - ta;
+    if (nt > 0) {
+      const int Z0 = A.tile_off[ta], Z1 = A.tile_off[tb];
+      for (int j = lane; j <= nt; j += 64) L.toff[j] = A.tile_off[ta + j] - Z0;
+      for (int j = lane; j < nt; j += 64) {
+        const int t = ta + j;
+        L.tcol[j] = A.tile_col[t] << 4;
+        L.trow[j] = (uint8_t)(A.tile_row[t] - ra);
+        if (MODE != kAll64) {
+          const int lv = tile_level<MODE>(A, seglev, t, force64);
+          L.tprec[j] = (uint8_t)lv;
+          L.tscale[j] = lv == MF_PREC_FP16 ? A.s16[t] : A.s8[t];
+          if (COUNT) cnt->add(lv);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      // Slices of 256 entries; the loads of the next slice are issued before
+      // the current one is consumed.
+      const int zb = Z0 & ~3;
+      const int nsl = (Z1 - zb + 255) >> 8;
+      SliceBuf cur, nxt;
+      slice_load<MODE>(A, L, zb, lane, Z0, Z1, nt, all64, cur);
+      for (int m = 0; m < nsl; ++m) {
+        if (m + 1 < nsl) slice_load<MODE>(A, L, zb + 256 * (m + 1), lane, Z0, Z1, nt, all64, nxt);
+        slice_consume<MODE>(A, L, cur, Z0, Z1, all64, x, x32);
+        cur = nxt;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    // Tile-rows [ra, R) are complete; tile-row R has started iff tile_ptr[R] < tb.
+    int R;
+    if (tb < T1) {
+      R = A.tile_row[tb];
+      if (R > rlim) R = rlim;
+    } else {
+      R = rlim;
+    }
+    for (int i = lane; i < (R - ra) * 16; i += 64) epi(ra * 16 + i, L.acc[i]);
+    const bool partial = R < tr1 && A.tile_ptr[R] < tb;
+    double carry = 0.0;
+    if (partial && lane < 16) carry = L.acc[(R - ra) * 16 + lane];
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane; i < kWinRows * 16; i += 64) L.acc[i] = 0.0;
+    __builtin_amdgcn_wave_barrier();
+    if (partial && lane < 16) L.acc[lane] = carry;
+    __builtin_amdgcn_wave_barrier();
+    ra = R;
+    ta = tb;
+  }
 }
 
 // Deterministic sum over the 64 lanes of a wave (fixed xor tree); every lane

@@ -26,8 +26,7 @@ struct DeviceLayout {
   int *tile_col = nullptr;     // tilenum
   int *tile_off = nullptr;     // tilenum+1 (payload offset; == csr_offset)
   int *tile_row = nullptr;     // tilenum (tile-row of each tile)
-  uint8_t *rowptr = nullptr;   // tilenum*16 row starts (rows >= h start at nnz)
-  uint8_t *col = nullptr;      // csrsize local columns
+  uint8_t *rc = nullptr;       // csrsize (+pad): (row << 4) | col inside the tile
   double *v64 = nullptr;       // csrsize FP64 payload
   float *v32 = nullptr;        // csrsize FP32 payload
   uint16_t *v16 = nullptr;     // csrsize FP16 payload (bits), scaled by s16
@@ -42,6 +41,10 @@ struct DeviceLayout {
   // learned policy parameters (FP16 bits), MF_MLP_NPARAM
   uint16_t *mlp16 = nullptr;
   int has_mlp = 0;
+  // stand-alone SpMV schedule: nnz-balanced tile-row ranges, one per wave
+  int *spmv_tr = nullptr;
+  int spmv_nwaves = 0;
+  int spmv_wave_nnz = 0;
 };
 
 struct Schedule {

@@ -20,6 +20,7 @@
 // i.e. iterative refinement inside one Krylov run).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "mf_device.h"
@@ -100,37 +101,31 @@ __device__ __forceinline__ void flush_hist(const PrecCount &c, unsigned long lon
 // True residual of the wave's tile-rows: r_new = b - A x (FP64), returned
 // per lane (lanes 0..15) and optionally stored.  Accumulates ||r_new||^2.
 __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int tr0, int tr1,
-                                                     int lane, double *store) {
+                                                     int lane, double *store, WaveLds *wl) {
   double acc = 0.0;
-  for (int bi = tr0; bi < tr1; ++bi) {
-    const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
-    if (lane < 16) {
-      const int i = bi * 16 + lane;
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+    [&](const int i, const double ax) {
       const double rv = a.b[i] - ax;
       if (store) store[i] = rv;
       acc += rv * rv;
-    }
-  }
+    });
   return acc;
 }
 
 // Restart helper: r = b - A x in FP64 for the wave's tile-rows, p = r (and
 // r^ = r for BiCGSTAB).  Returns the wave's share of ||r||^2.
 __device__ __forceinline__ double restart_rows(const SolveArgs &a, int tr0, int tr1, int lane,
-                                               bool mixed, bool bicg) {
+                                               bool mixed, bool bicg, WaveLds *wl) {
   double acc = 0.0;
-  for (int bi = tr0; bi < tr1; ++bi) {
-    const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
-    if (lane < 16) {
-      const int i = bi * 16 + lane;
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+    [&](const int i, const double ax) {
       const double rv = a.b[i] - ax;
       a.r[i] = rv;
       a.p[i] = rv;
       if (mixed) a.p32[i] = (float)rv;
       if (bicg) a.rh[i] = rv;
       acc += rv * rv;
-    }
-  }
+    });
   return acc;
 }
 
@@ -138,6 +133,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   __shared__ double lds[kWavesPerBlock];
   __shared__ int lds_i;
+  __shared__ WaveLds wl[kWavesPerBlock];
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const int tr0 = a.wave_tr[wid], tr1 = a.wave_tr[wid + 1];
@@ -149,18 +145,15 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
 
   // r = b - A x0, p = r.
   double red[2] = {0.0, 0.0};
-  for (int bi = tr0; bi < tr1; ++bi) {
-    const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
-    if (lane < 16) {
-      const int i = bi * 16 + lane;
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+    [&](const int i, const double ax) {
       const double bv = a.b[i], rv = bv - ax;
       a.r[i] = rv;
       a.p[i] = rv;
       if (mixed) a.p32[i] = (float)rv;
       red[0] += rv * rv;
       red[1] += bv * bv;
-    }
-  }
+    });
   grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
   double rr = red[0];
   const double bb = red[1];
@@ -173,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   while (!done) {
     if (restart) {
       // Fallback: FP64 residual, p = r, every tile FP64 from here on.
-      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, false)};
+      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, false, wl)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
       rr = tr[0];
@@ -185,14 +178,11 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
     ++it;
     // q = A p, fused with p.q for the wave's rows.
     double pq[1] = {0.0};
-    for (int bi = tr0; bi < tr1; ++bi) {
-      const double qv = tile_row_sum<MODE, mixed>(a.A, bi, lane, a.p, a.p32, a.seg_p, force64, &cnt);
-      if (lane < 16) {
-        const int i = bi * 16 + lane;
+    wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
+      [&](const int i, const double qv) {
         a.q[i] = qv;
         pq[0] += a.p[i] * qv;
-      }
-    }
+      });
     grid_sum<1>(pq, a.partials, a.gs, epoch, slot, lds, &lds_i);
     if (mixed && !force64 && !(pq[0] > 0.0 && isfinite(pq[0]))) {
       // The low-precision operator lost positive definiteness.
@@ -222,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       if (converged || periodic) {
         // FP64 recomputation of the residual (x is complete: the grid sum
         // above is a barrier after the x update).
-        double tr[1] = {true_residual_rows(a, tr0, tr1, lane, a.r)};
+        double tr[1] = {true_residual_rows(a, tr0, tr1, lane, a.r, wl)};
         grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
         ++refresh;
         it_ref = it;
@@ -233,6 +223,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           force64 = 1;
           restart = true;
         }
+        // A recursive residual far below the true one makes beta meaningless.
+        if (tr[0] > 4.0 * rr_new) restart = true;
         rr_new = tr[0];
         rr_ref = tr[0];
       }
@@ -262,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   }
   if (mixed) flush_hist(cnt, a.hist, lds);
   // Final FP64 residual for the report.
-  double fin[1] = {true_residual_rows(a, tr0, tr1, lane, nullptr)};
+  double fin[1] = {true_residual_rows(a, tr0, tr1, lane, nullptr, wl)};
   grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.stats[kStIters] = it;
@@ -287,6 +279,7 @@ template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   __shared__ double lds[kWavesPerBlock];
   __shared__ int lds_i;
+  __shared__ WaveLds wl[kWavesPerBlock];
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const int tr0 = a.wave_tr[wid], tr1 = a.wave_tr[wid + 1];
@@ -298,10 +291,8 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   double *v = a.q, *t = a.tv;
 
   double red[2] = {0.0, 0.0};
-  for (int bi = tr0; bi < tr1; ++bi) {
-    const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
-    if (lane < 16) {
-      const int i = bi * 16 + lane;
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+    [&](const int i, const double ax) {
       const double bv = a.b[i], rv = bv - ax;
       a.r[i] = rv;
       a.rh[i] = rv;
@@ -309,8 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
       if (mixed) a.p32[i] = (float)rv;
       red[0] += rv * rv;
       red[1] += bv * bv;
-    }
-  }
+    });
   grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
   double rho = red[0], rr = red[0];
   const double bb = red[1];
@@ -323,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   while (!done) {
     if (restart) {
       if (++restarts > kMaxRestart) { status = MF_ERR_BREAKDOWN; break; }
-      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, true)};
+      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, true, wl)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
       rr = rho = tr[0];
@@ -335,14 +325,11 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     ++it;
     // v = A p, v.rh
     double vr[1] = {0.0};
-    for (int bi = tr0; bi < tr1; ++bi) {
-      const double vv = tile_row_sum<MODE, mixed>(a.A, bi, lane, a.p, a.p32, a.seg_p, force64, &cnt);
-      if (lane < 16) {
-        const int i = bi * 16 + lane;
+    wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
+      [&](const int i, const double vv) {
         v[i] = vv;
         vr[0] += vv * a.rh[i];
-      }
-    }
+      });
     grid_sum<1>(vr, a.partials, a.gs, epoch, slot, lds, &lds_i);
     if (!(vr[0] != 0.0) || !isfinite(vr[0])) {
       if (mixed) force64 = 1;
@@ -370,15 +357,12 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     grid_sync(a.gs, epoch, &lds_i);
     // t = A s, t.s and t.t
     double tw[2] = {0.0, 0.0};
-    for (int bi = tr0; bi < tr1; ++bi) {
-      const double tvv = tile_row_sum<MODE, mixed>(a.A, bi, lane, a.s, a.s32, a.seg_s, force64, &cnt);
-      if (lane < 16) {
-        const int i = bi * 16 + lane;
+    wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.s, a.s32, a.seg_s, force64, wl[threadIdx.x >> 6], lane, &cnt,
+      [&](const int i, const double tvv) {
         t[i] = tvv;
         tw[0] += tvv * a.s[i];
         tw[1] += tvv * tvv;
-      }
-    }
+      });
     grid_sum<2>(tw, a.partials, a.gs, epoch, slot, lds, &lds_i);
     const double omega = tw[1] != 0.0 && isfinite(tw[1]) ? tw[0] / tw[1] : 0.0;
     // x += alpha p + omega s, r = s - omega t, rho' = r.rh, ||r||^2
@@ -400,16 +384,13 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
                             (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
       if (converged || periodic || omega == 0.0 || !isfinite(rr_new)) {
         double tr[2] = {0.0, 0.0};
-        for (int bi = tr0; bi < tr1; ++bi) {
-          const double ax = tile_row_sum<kAll64>(a.A, bi, lane, a.x, nullptr);
-          if (lane < 16) {
-            const int i = bi * 16 + lane;
+        wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+          [&](const int i, const double ax) {
             const double rv = a.b[i] - ax;
             a.r[i] = rv;
             tr[0] += rv * rv;
             tr[1] += rv * a.rh[i];
-          }
-        }
+          });
         grid_sum<2>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
         ++refresh;
         it_ref = it;
@@ -449,7 +430,258 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     grid_sync(a.gs, epoch, &lds_i);
   }
   if (mixed) flush_hist(cnt, a.hist, lds);
-  double fin[1] = {true_residual_rows(a, tr0, tr1, lane, nullptr)};
+  double fin[1] = {true_residual_rows(a, tr0, tr1, lane, nullptr, wl)};
+  grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.stats[kStIters] = it;
+    a.stats[kStRefresh] = refresh;
+    a.stats[kStFallback] = force64;
+    a.stats[kStRecursive] = rr;
+    a.stats[kStTrue] = fin[0];
+    a.stats[kStBB] = bb;
+    a.stats[kStStatus] = status;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident CG (Chronopoulos-Gear form).
+//
+// Same mathematics as textbook CG with one SpMV (w = A r) and one fused pair
+// of dot products (gamma = r.r, delta = w.r) per iteration, so an iteration
+// needs two grid syncs instead of three:
+//   beta = gamma/gamma_old, alpha = gamma/(delta - beta*gamma/alpha_old),
+//   p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s   (s = A p).
+// Every wave owns 16*(tr1-tr0) <= 64*K consecutive rows and keeps p and s of
+// those rows in registers (lane L holds rows L, L+64, ...) and x in
+// wave-private LDS; the SpMV epilogue drops w into wave-private LDS.  The only
+// vector stored per iteration is r (plus its FP32 copy for low-precision
+// tiles), which the next SpMV gathers; the wave reads its own rows of r back
+// (L2-resident: the SpMV has just gathered them) instead of holding a third
+// register vector.  FP64 residual recomputation replaces r by b - A x and
+// s by A p, both in FP64, which keeps the recurrences consistent without
+// restarting (residual replacement); a stagnating mixed run falls back to
+// FP64 tiles and restarts from the recomputed residual.
+// ---------------------------------------------------------------------------
+constexpr int kResBlock = 512;  // one block per CU, 8 waves, up to 256 VGPRs
+constexpr int kResWaves = kResBlock / 64;
+constexpr int kResK = 16;        // rows per lane: up to 1024 rows per wave
+
+// A copy of v the compiler cannot see through: per-row addresses derived from
+// it are recomputed where they are used instead of being hoisted out of the
+// solver loop (sixteen hoisted 64-bit addresses per vector would not fit in
+// the register budget of a resident wave).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+template <int MODE, int K>
+__global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
+  __shared__ double lds[kResWaves];
+  __shared__ int lds_i;
+  __shared__ WaveLds wl[kResWaves];
+  __shared__ double wbuf[kResWaves * K * 64];
+  __shared__ double xbuf[kResWaves * K * 64];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wid = blockIdx.x * kResWaves + wv;
+  const int tr0 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid]);
+  const int tr1 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid + 1]);
+  const int row0 = tr0 * 16;
+  const int nrows = (tr1 - tr0) * 16;
+  double *W = wbuf + wv * K * 64;
+  double *X = xbuf + wv * K * 64;  // x of the wave's rows
+  WaveLds &L = wl[wv];
+  constexpr bool mixed = MODE != kAll64;
+  unsigned epoch = 0;
+  int slot = 0;
+  PrecCount cnt;
+  double p[K], s[K];
+  auto epi_w = [&](const int i, const double v) { W[i - row0] = v; };
+
+  // r = b - A x0
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, L, lane, nullptr, epi_w);
+  __builtin_amdgcn_wave_barrier();
+  double red[2] = {0.0, 0.0};
+  const int ln0 = opaque(lane);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int li = ln0 + 64 * k;
+    p[k] = s[k] = 0.0;
+    if (li < nrows) {
+      const int i = row0 + li;
+      const double bv = a.b[i];
+      X[li] = a.x[i];
+      const double rv = bv - W[li];
+      red[0] += rv * rv;
+      red[1] += bv * bv;
+      a.r[i] = rv;
+      if (mixed) a.p32[i] = (float)rv;
+    }
+  }
+  grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
+  double rr = red[0];
+  const double bb = red[1];
+  const double stop2 = a.tol2 * bb;
+  double rr_ref = rr, gamma_old = 1.0, alpha_old = 1.0;
+  bool first = true;
+  int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0;
+  bool done = bb == 0.0 || (a.fixed_iters <= 0 && rr <= stop2);
+
+  bool pending = false;  // recompute the FP64 residual before the next step
+  while (!done) {
+    double gamma = 0.0, delta = 0.0;
+    if (!pending) {
+      // w = A r (r published by every wave before the last sync)
+      wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.r, a.p32, a.seg_p, force64, L, lane, &cnt, epi_w);
+      __builtin_amdgcn_wave_barrier();
+      double gd[2] = {0.0, 0.0};
+      const int ln1 = opaque(lane);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int li = ln1 + 64 * k;
+        if (li < nrows) {
+          const double rv = a.r[row0 + li];
+          gd[0] += rv * rv;
+          gd[1] += W[li] * rv;
+        }
+      }
+      grid_sum<2>(gd, a.partials, a.gs, epoch, slot, lds, &lds_i);
+      gamma = gd[0];
+      delta = gd[1];
+      rr = gamma;
+      if (a.fixed_iters <= 0) {
+        const bool converged = gamma <= stop2;
+        const bool periodic = mixed && !force64 &&
+                              (gamma <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
+        pending = converged || periodic;
+        if (!pending && it >= a.maxiter) { status = MF_NOT_CONVERGED; break; }
+      }
+    }
+    if (pending) {
+      pending = false;
+      // FP64 residual replacement: publish x, r = b - A x.
+      const int ln2 = opaque(lane);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int li = ln2 + 64 * k;
+        if (li < nrows) a.x[row0 + li] = X[li];
+      }
+      grid_sync(a.gs, epoch, &lds_i);
+      wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, L, lane, nullptr, epi_w);
+      __builtin_amdgcn_wave_barrier();
+      double tr[1] = {0.0};
+      const int ln3 = opaque(lane);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int li = ln3 + 64 * k;
+        if (li < nrows) {
+          const int i = row0 + li;
+          const double rv = a.b[i] - W[li];
+          tr[0] += rv * rv;
+          a.r[i] = rv;
+          if (mixed) a.p32[i] = (float)rv;
+        }
+      }
+      grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
+      ++refresh;
+      it_ref = it;
+      // A recursive residual far below the true one (a low-precision operator
+      // or rounding drifted it) leaves beta = gamma/gamma_old meaningless:
+      // restart the search direction from the recomputed residual.
+      if (tr[0] > 4.0 * rr) first = true;
+      rr = tr[0];
+      if (tr[0] <= stop2) break;
+      if (it >= a.maxiter) { status = MF_NOT_CONVERGED; break; }
+      if (mixed && !force64 && (tr[0] > 0.25 * rr_ref || refresh > a.max_refresh)) {
+        force64 = 1;
+        first = true;  // stagnation: FP64 tiles from here, restart the recurrence
+      }
+      rr_ref = tr[0];
+      if (!first) {
+        // s = A p in FP64 so that s and the new r stay consistent
+        const int ln4 = opaque(lane);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int li = ln4 + 64 * k;
+          if (li < nrows) a.q[row0 + li] = p[k];
+        }
+        grid_sync(a.gs, epoch, &lds_i);
+        wave_spmv<kAll64, false>(a.A, tr0, tr1, a.q, nullptr, nullptr, 0, L, lane, nullptr, epi_w);
+        __builtin_amdgcn_wave_barrier();
+        const int ln5 = opaque(lane);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int li = ln5 + 64 * k;
+          if (li < nrows) s[k] = W[li];
+        }
+      }
+      continue;
+    }
+    double beta, alpha;
+    if (first) {
+      beta = 0.0;
+      alpha = gamma / delta;
+    } else {
+      beta = gamma / gamma_old;
+      alpha = gamma / (delta - beta * gamma / alpha_old);
+    }
+    if (!isfinite(alpha) || alpha == 0.0 || (mixed && !force64 && !(alpha > 0.0))) {
+      if (mixed && !force64) {
+        // The low-precision operator lost definiteness and r has been
+        // updated with it: FP64 tiles, recompute r = b - A x, restart.
+        force64 = 1;
+        first = true;
+        pending = true;
+        continue;
+      }
+      status = MF_ERR_BREAKDOWN;
+      break;
+    }
+    first = false;
+    gamma_old = gamma;
+    alpha_old = alpha;
+    ++it;
+    const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
+    const int ln6 = opaque(lane);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int li = ln6 + 64 * k;
+      double d = 0.0;
+      if (li < nrows) {
+        const int i = row0 + li;
+        const double rv = a.r[i];
+        p[k] = rv + beta * p[k];
+        s[k] = W[li] + beta * s[k];
+        X[li] += alpha * p[k];
+        const double rn = rv - alpha * s[k];
+        d = fabs(rn - rv);
+        a.r[i] = rn;
+        if (mixed) a.p32[i] = (float)rn;
+      }
+      if (upd) {
+        d = group16_max(d);
+        if ((lane & 15) == 0 && li < nrows) a.seg_p[(row0 + li) >> 4] = (uint8_t)rule_level(d);
+      }
+    }
+    grid_sync(a.gs, epoch, &lds_i);
+    if (a.fixed_iters > 0 && it >= a.fixed_iters) break;
+  }
+  if (mixed) flush_hist(cnt, a.hist, lds);
+  // publish x, final FP64 residual for the report
+  const int ln7 = opaque(lane);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int li = ln7 + 64 * k;
+    if (li < nrows) a.x[row0 + li] = X[li];
+  }
+  grid_sync(a.gs, epoch, &lds_i);
+  double fin[1] = {0.0};
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, L, lane, nullptr,
+                           [&](const int i, const double v) {
+                             const double d = a.b[i] - v;
+                             fin[0] += d * d;
+                           });
   grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.stats[kStIters] = it;
@@ -530,11 +762,12 @@ mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
 }
 
 template <typename K>
-mf_status grid_for(K kernel, int override_blocks, int *G) {
+mf_status grid_for(K kernel, int override_blocks, int *G, int block = kBlock, bool one_per_cu = false) {
   int dev = 0, ncu = 0, nb = 0;
   MF_HIP(hipGetDevice(&dev));
   MF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, 0));
+  MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0));
+  if (one_per_cu && nb > 1) nb = 1;
   if (nb < 1) return fail(MF_ERR_COOP, "persistent kernel does not fit on a CU");
   const int maxg = nb * ncu;
   *G = override_blocks > 0 ? std::min(override_blocks, maxg) : maxg;
@@ -542,15 +775,26 @@ mf_status grid_for(K kernel, int override_blocks, int *G) {
   return MF_OK;
 }
 
-mf_status ensure_schedule(mf_tiled *t, int nwaves) {
-  if (t->sched.wave_tr && t->sched_waves_for == nwaves) return MF_OK;
+// Per-wave tile-row ranges: nnz-balanced (memory-resident kernels) or an
+// even split of tile-rows (register-resident CG, whose per-wave state is
+// sized by rows).  Cached per (wave count, kind).
+mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
+  const int key = 2 * nwaves + (even_rows ? 1 : 0);
+  if (t->sched.wave_tr && t->sched_waves_for == key) return MF_OK;
   if (t->sched.wave_tr) MF_HIP(hipFree(t->sched.wave_tr));
   t->sched.wave_tr = nullptr;
-  std::vector<int> b = balance_tilerows(t, nwaves);
+  std::vector<int> b;
+  if (even_rows) {
+    const int q = (t->tilem + nwaves - 1) / nwaves;
+    b.resize(nwaves + 1);
+    for (int w = 0; w <= nwaves; ++w) b[w] = std::min(t->tilem, w * q);
+  } else {
+    b = balance_tilerows(t, nwaves);
+  }
   MF_HIP(hipMalloc(&t->sched.wave_tr, sizeof(int) * (nwaves + 1)));
   MF_HIP(hipMemcpy(t->sched.wave_tr, b.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice));
   t->sched.nwaves = nwaves;
-  t->sched_waves_for = nwaves;
+  t->sched_waves_for = key;
   return MF_OK;
 }
 
@@ -586,17 +830,37 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   const int mode = spmv_mode_for(opt.policy);
 
   const void *kern;
-  if (bicg)
-    kern = mode == kAll64 ? (const void *)k_bicgstab<kAll64>
-         : mode == kPerSeg ? (const void *)k_bicgstab<kPerSeg> : (const void *)k_bicgstab<kPerTile>;
-  else
-    kern = mode == kAll64 ? (const void *)k_cg<kAll64>
-         : mode == kPerSeg ? (const void *)k_cg<kPerSeg> : (const void *)k_cg<kPerTile>;
-  int G = 0;
-  mf_status sst = grid_for(kern, opt.blocks, &G);
-  if (sst != MF_OK) return sst;
-  sst = ensure_schedule(t, G * kWavesPerBlock);
-  if (sst != MF_OK) return sst;
+  int G = 0, block = kBlock;
+  mf_status sst = MF_OK;
+  bool resident = false;
+  if (!bicg && std::getenv("MF_NO_RESIDENT") == nullptr) {
+    // Register-resident CG when every wave's even share of rows fits 64*K.
+    kern = mode == kAll64 ? (const void *)k_cg_res<kAll64, kResK>
+         : mode == kPerSeg ? (const void *)k_cg_res<kPerSeg, kResK>
+                           : (const void *)k_cg_res<kPerTile, kResK>;
+    sst = grid_for(kern, opt.blocks, &G, kResBlock, true);
+    if (sst != MF_OK) return sst;
+    const int nw = G * kResWaves;
+    const int q = (t->tilem + nw - 1) / nw;
+    if (16 * q <= 64 * kResK) {
+      resident = true;
+      block = kResBlock;
+      sst = ensure_schedule(t, nw, true);
+      if (sst != MF_OK) return sst;
+    }
+  }
+  if (!resident) {
+    if (bicg)
+      kern = mode == kAll64 ? (const void *)k_bicgstab<kAll64>
+           : mode == kPerSeg ? (const void *)k_bicgstab<kPerSeg> : (const void *)k_bicgstab<kPerTile>;
+    else
+      kern = mode == kAll64 ? (const void *)k_cg<kAll64>
+           : mode == kPerSeg ? (const void *)k_cg<kPerSeg> : (const void *)k_cg<kPerTile>;
+    sst = grid_for(kern, opt.blocks, &G);
+    if (sst != MF_OK) return sst;
+    sst = ensure_schedule(t, G * kWavesPerBlock);
+    if (sst != MF_OK) return sst;
+  }
   Workspace w;
   sst = get_workspace(t, G, w);
   if (sst != MF_OK) return sst;
@@ -650,10 +914,10 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   // (e.g. another process holds the device's cooperative queue) the same grid
   // is launched normally: the barrier needs only atomics, and its bounded
   // spin turns a block that is not resident into a reported abort.
-  hipError_t le = hipLaunchCooperativeKernel(kern, dim3(G), dim3(kBlock), args, 0, st);
+  hipError_t le = hipLaunchCooperativeKernel(kern, dim3(G), dim3(block), args, 0, st);
   if (le == hipErrorIllegalState || le == hipErrorNotSupported) {
     (void)hipGetLastError();
-    le = hipLaunchKernel(kern, dim3(G), dim3(kBlock), args, 0, st);
+    le = hipLaunchKernel(kern, dim3(G), dim3(block), args, 0, st);
   }
   if (le != hipSuccess) {
     (void)hipEventDestroy(e0);

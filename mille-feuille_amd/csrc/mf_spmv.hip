@@ -1,26 +1,29 @@
-// mf_spmv.hip -- stand-alone tiled SpMV y = A x (wave64, one wave per
-// tile-row, no atomics).
+// mf_spmv.hip -- stand-alone tiled SpMV y = A x (streamed, wave64, no global
+// atomics).
 //
-// Launch shape: 256-thread blocks (4 waves = 4 tile-rows).  Blocks are
-// dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8), so the
-// block index is remapped such that each XCD walks one contiguous eighth of
-// the tile-rows: neighbouring tile-rows share x segments, and keeping them on
-// one XCD keeps those segments in that XCD's L2.
+// Launch shape: 256-thread blocks, one nnz-balanced range of tile-rows per
+// wave (about 1024 entries).  Blocks are dispatched round-robin over the 8
+// XCDs (block b runs on XCD b % 8), so the block index is remapped such that
+// each XCD walks one contiguous eighth of the matrix: neighbouring tile-rows
+// share x segments, and keeping them on one XCD keeps those segments in that
+// XCD's L2.
 #include "mf_device.h"
 
 namespace mf {
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_spmv(TileView A, const double *__restrict__ x,
+__global__ __launch_bounds__(256) void k_spmv(TileView A, const int *__restrict__ wave_tr,
+                                              int nwaves, const double *__restrict__ x,
                                               const float *__restrict__ x32,
                                               double *__restrict__ y) {
+  __shared__ WaveLds wl[4];
   const int per_xcd = gridDim.x >> 3;  // gridDim.x is a multiple of 8
   const int lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-  const int bi = lb * 4 + (threadIdx.x >> 6);
+  const int w = lb * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (bi >= A.tilem) return;
-  const double v = tile_row_sum<MODE>(A, bi, lane, x, x32);
-  if (lane < 16) y[bi * 16 + lane] = v;
+  if (w >= nwaves) return;
+  wave_spmv<MODE, false>(A, wave_tr[w], wave_tr[w + 1], x, x32, nullptr, 0, wl[threadIdx.x >> 6],
+                         lane, nullptr, [&](const int i, const double v) { y[i] = v; });
 }
 
 // Zero the padding of y beyond the tile-rows (npad may exceed 16*tilem when
@@ -33,12 +36,13 @@ __global__ void k_zero_tail(double *y, int from, int to) {
 mf_status launch_spmv(const mf_tiled *t, const double *d_x, const float *d_x32, double *d_y,
                       int mode, hipStream_t s) {
   const TileView A = view_of(t);
-  const int blocks = ((A.tilem + 3) / 4 + 7) / 8 * 8;
-  if (A.tilem > 0) {
+  const int nw = t->dev.spmv_nwaves;
+  const int blocks = ((nw + 3) / 4 + 7) / 8 * 8;
+  if (A.tilem > 0 && nw > 0) {
     if (mode == kAll64)
-      k_spmv<kAll64><<<blocks, 256, 0, s>>>(A, d_x, d_x32, d_y);
+      k_spmv<kAll64><<<blocks, 256, 0, s>>>(A, t->dev.spmv_tr, nw, d_x, d_x32, d_y);
     else
-      k_spmv<kPerTile><<<blocks, 256, 0, s>>>(A, d_x, d_x32, d_y);
+      k_spmv<kPerTile><<<blocks, 256, 0, s>>>(A, t->dev.spmv_tr, nw, d_x, d_x32, d_y);
     MF_HIP(hipGetLastError());
   }
   const int from = 16 * A.tilem, to = A.npad;
@@ -49,9 +53,37 @@ mf_status launch_spmv(const mf_tiled *t, const double *d_x, const float *d_x32, 
   return MF_OK;
 }
 
+__global__ __launch_bounds__(256) void k_stream_copy(const double4 *__restrict__ src,
+                                                     double4 *__restrict__ dst, int64_t n4) {
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n4; i += (int64_t)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
+__global__ void k_copy_tail(const double *__restrict__ src, double *__restrict__ dst, int64_t from,
+                            int64_t n) {
+  const int64_t i = from + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 }  // namespace mf
 
 using namespace mf;
+
+extern "C" mf_status mf_stream_copy(const double *d_src, double *d_dst, int64_t n, void *stream) {
+  if (!d_src || !d_dst || n < 0) return fail(MF_ERR_ARG, "mf_stream_copy: bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n4 = n / 4;
+  if (((uintptr_t)d_src | (uintptr_t)d_dst) & 31) return fail(MF_ERR_ARG, "mf_stream_copy: 32-byte alignment");
+  if (n4 > 0) {
+    k_stream_copy<<<2048, 256, 0, s>>>((const double4 *)d_src, (double4 *)d_dst, n4);
+    MF_HIP(hipGetLastError());
+  }
+  if (n > 4 * n4) {
+    k_copy_tail<<<1, 64, 0, s>>>(d_src, d_dst, 4 * n4, n);
+    MF_HIP(hipGetLastError());
+  }
+  return MF_OK;
+}
 
 extern "C" mf_status mf_spmv(mf_tiled *t, const double *d_x, const float *d_x32, double *d_y,
                              int mode, void *stream) {

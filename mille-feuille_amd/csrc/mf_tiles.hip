@@ -7,17 +7,20 @@
 //   tile_col  T i32            tile column
 //   tile_off  (T+1) i32        payload offset of each tile (Z at the end)
 //   tile_row  T i32            tile-row of each tile
-//   rowptr    16T u8           per tile, start offset of each of its 16 rows
-//   col       Z u8             tile-local column
+//   rc        Z u8             (row << 4) | col of the entry inside its tile
 //   v64/v32/v16/v8             Z entries each: FP64, FP32, FP16 (scaled by
-//                              a power of two per tile), INT8 (max/127 scale)
+//                              a power of two per tile), INT8 (scaled by a
+//                              power of two per tile)
 //   s16, s8   T f32            per-tile scales
 //   qerr      3T f64           per-tile max |a - dequant(a)| for FP32/16/INT8
 //   amax      T f64            per-tile max |a|
 //   prec      T u8             current precision level of each tile
-// The reference's row pointers are 8-bit row *starts* (src/csr2block.h:252),
-// which never exceed 240, so 8 bits suffice for any tile; row ends come from
-// the next row's start or the tile's nnz.
+//   spmv_tr                    nnz-balanced tile-row ranges of the
+//                              stand-alone SpMV waves
+// Entries keep the reference's order (tile by tile, row by row inside a
+// tile, src/csr2block.h:239-252); the per-entry row nibble replaces the
+// reference's 16 row pointers per tile on the device, so one SpMV reads
+// value + 1 byte per entry and 12-16 bytes per tile.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -25,6 +28,11 @@
 #include "mf_device.h"
 
 namespace mf {
+
+// Payload arrays carry kPad extra zero entries: the streamed SpMV reads
+// entries in aligned groups of four and may touch up to three past the end.
+constexpr int kPad = 64;
+constexpr int kSpmvWaveNnz = 512;
 
 __global__ void k_pack_payloads(int T, const int *__restrict__ off, const double *__restrict__ v64,
                                 float *__restrict__ v32, uint16_t *__restrict__ v16,
@@ -100,33 +108,40 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   d.npad = kTile * std::max(t->tilem, t->tilen);
   if (d.npad == 0) d.npad = kTile;
 
+  // Per entry: (row << 4) | col inside the tile, from the tile's row starts.
   std::vector<int> tile_row(T);
-  std::vector<uint8_t> rp((size_t)T * kTile);
+  std::vector<uint8_t> rc((size_t)Z + kPad, 0);
   std::vector<int> diag(TM, -1);
   for (int bi = 0; bi < TM; ++bi) {
     const int h = bi == TM - 1 ? t->rowA - (TM - 1) * kTile : kTile;
     for (int tt = t->tile_ptr[bi]; tt < t->tile_ptr[bi + 1]; ++tt) {
       tile_row[tt] = bi;
       if (t->tile_columnidx[tt] == bi) diag[bi] = tt;
-      const int nzt = t->csr_offset[tt + 1] - t->csr_offset[tt];
-      for (int r = 0; r < kTile; ++r)
-        rp[(size_t)tt * kTile + r] = r < h ? t->Blockcsr_Ptr[t->csrptr_offset[tt] + r] : (uint8_t)std::min(nzt, 255);
+      const int off = t->csr_offset[tt], nzt = t->csr_offset[tt + 1] - off;
+      const unsigned char *rp = &t->Blockcsr_Ptr[t->csrptr_offset[tt]];
+      for (int r = 0; r < h; ++r) {
+        const int e = r == h - 1 ? nzt : rp[r + 1];
+        for (int k = rp[r]; k < e; ++k)
+          rc[off + k] = (uint8_t)((r << 4) | t->Tile_csr_Col[off + k]);
+      }
     }
   }
-  // A 256-entry tile (16 rows x 16 columns) has row 15 starting at 240 and
-  // ends at nnz = 256, which the SpMV reads from tile_off; only padded rows
-  // of a short last tile-row store the tile nnz here, and those tiles have at
-  // most h*16 < 256 entries.
+  // nnz-balanced tile-row ranges for the stand-alone SpMV, about
+  // kSpmvWaveNnz entries per wave (retuned by mf_tile_set_schedule).
+  const int nw = std::max(1, (int)((Z + kSpmvWaveNnz - 1) / kSpmvWaveNnz));
+  std::vector<int> wtr = balance_tilerows(t, nw);
+  d.spmv_wave_nnz = kSpmvWaveNnz;
   MF_TRY(dalloc(&d.tile_ptr, TM + 1));
   MF_TRY(dalloc(&d.tile_col, T));
   MF_TRY(dalloc(&d.tile_off, T + 1));
   MF_TRY(dalloc(&d.tile_row, T));
-  MF_TRY(dalloc(&d.rowptr, (size_t)T * kTile));
-  MF_TRY(dalloc(&d.col, Z));
-  MF_TRY(dalloc(&d.v64, Z));
-  MF_TRY(dalloc(&d.v32, Z));
-  MF_TRY(dalloc(&d.v16, Z));
-  MF_TRY(dalloc(&d.v8, Z));
+  MF_TRY(dalloc(&d.rc, (size_t)Z + kPad));
+  MF_TRY(dalloc(&d.v64, (size_t)Z + kPad));
+  MF_TRY(dalloc(&d.v32, (size_t)Z + kPad));
+  MF_TRY(dalloc(&d.v16, (size_t)Z + kPad));
+  MF_TRY(dalloc(&d.v8, (size_t)Z + kPad));
+  MF_TRY(dalloc(&d.spmv_tr, nw + 1));
+  d.spmv_nwaves = nw;
   MF_TRY(dalloc(&d.s16, T));
   MF_TRY(dalloc(&d.s8, T));
   MF_TRY(dalloc(&d.qerr, 3 * (size_t)T));
@@ -139,9 +154,13 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_HIP(hipMemcpy(d.tile_col, t->tile_columnidx.data(), sizeof(int) * T, hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.tile_off, t->csr_offset.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.tile_row, tile_row.data(), sizeof(int) * T, hipMemcpyHostToDevice));
-  MF_HIP(hipMemcpy(d.rowptr, rp.data(), (size_t)T * kTile, hipMemcpyHostToDevice));
-  MF_HIP(hipMemcpy(d.col, t->Tile_csr_Col.data(), Z, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.rc, rc.data(), (size_t)Z + kPad, hipMemcpyHostToDevice));
+  MF_HIP(hipMemset(d.v64, 0, sizeof(double) * ((size_t)Z + kPad)));
+  MF_HIP(hipMemset(d.v32, 0, sizeof(float) * ((size_t)Z + kPad)));
+  MF_HIP(hipMemset(d.v16, 0, sizeof(uint16_t) * ((size_t)Z + kPad)));
+  MF_HIP(hipMemset(d.v8, 0, (size_t)Z + kPad));
   MF_HIP(hipMemcpy(d.v64, t->Blockcsr_Val.data(), sizeof(double) * Z, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.spmv_tr, wtr.data(), sizeof(int) * (nw + 1), hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.diag_tile, diag.data(), sizeof(int) * TM, hipMemcpyHostToDevice));
   MF_HIP(hipMemset(d.prec, MF_PREC_FP64, T));
   MF_HIP(hipMemset(d.prec_user, MF_PREC_FP64, T));
@@ -160,7 +179,7 @@ mf_status upload_tiles(mf_tiled *t, int device) {
 void free_device(mf_tiled *t) {
   DeviceLayout &d = t->dev;
   if (d.device >= 0) (void)hipSetDevice(d.device);
-  void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rowptr, d.col,
+  void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.spmv_tr,
                   d.v64,      d.v32,      d.v16,      d.v8,       d.s16,    d.s8,
                   d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp16,
                   t->sched.wave_tr, t->work};
@@ -248,5 +267,29 @@ extern "C" mf_status mf_tile_get_quant_error(const mf_tiled *t, double *err) {
   if (t->dev.device < 0) return fail(MF_ERR_ARG, "mf_tile_get_quant_error: host-only handle");
   MF_HIP(hipSetDevice(t->dev.device));
   MF_HIP(hipMemcpy(err, t->dev.qerr, sizeof(double) * 3 * (size_t)t->tilenum, hipMemcpyDeviceToHost));
+  return MF_OK;
+}
+
+extern "C" mf_status mf_tile_set_schedule(mf_tiled *t, int spmv_wave_nnz) {
+  if (!t) return fail(MF_ERR_ARG, "mf_tile_set_schedule: null handle");
+  if (t->dev.device < 0) return fail(MF_ERR_ARG, "mf_tile_set_schedule: host-only handle");
+  if (spmv_wave_nnz == 0) return MF_OK;
+  if (spmv_wave_nnz < 64 || spmv_wave_nnz > (1 << 20))
+    return fail(MF_ERR_ARG, "mf_tile_set_schedule: spmv_wave_nnz out of [64, 2^20]");
+  MF_HIP(hipSetDevice(t->dev.device));
+  const int nw = std::max(1, (int)(((int64_t)t->csrsize + spmv_wave_nnz - 1) / spmv_wave_nnz));
+  std::vector<int> wtr = balance_tilerows(t, nw);
+  int *p = nullptr;
+  MF_HIP(hipMalloc(&p, sizeof(int) * (nw + 1)));
+  hipError_t e = hipMemcpy(p, wtr.data(), sizeof(int) * (nw + 1), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return fail(MF_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
+  MF_HIP(hipDeviceSynchronize());
+  if (t->dev.spmv_tr) MF_HIP(hipFree(t->dev.spmv_tr));
+  t->dev.spmv_tr = p;
+  t->dev.spmv_nwaves = nw;
+  t->dev.spmv_wave_nnz = spmv_wave_nnz;
   return MF_OK;
 }

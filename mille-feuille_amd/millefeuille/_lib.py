@@ -99,6 +99,8 @@ SIGNATURES = {
     "mf_tile_get_scales": (C.c_int, [_vp, _vp, _vp]),
     "mf_tile_get_quant_error": (C.c_int, [_vp, _vp]),
     "mf_spmv": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "mf_stream_copy": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
+    "mf_tile_set_schedule": (C.c_int, [_vp, C.c_int]),
     "mf_solve_opts_default": (None, [C.POINTER(MfSolveOpts)]),
     "mf_cg_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(MfSolveOpts), C.POINTER(MfReport)]),
     "mf_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(MfSolveOpts), C.POINTER(MfReport)]),

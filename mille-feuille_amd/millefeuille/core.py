@@ -142,6 +142,9 @@ class TiledMatrix:
             raise ValueError("one level per tile expected")
         check("mf_tile_set_precision", lib.mf_tile_set_precision(self.h, _p(lv), int(uniform)))
 
+    def set_schedule(self, spmv_wave_nnz):
+        check("mf_tile_set_schedule", lib.mf_tile_set_schedule(self.h, int(spmv_wave_nnz)))
+
     def get_precision(self):
         out = np.zeros(max(self.tilenum, 1), np.uint8)
         check("mf_tile_get_precision", lib.mf_tile_get_precision(self.h, _p(out)))

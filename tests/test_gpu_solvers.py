@@ -57,7 +57,7 @@ def _solve(mf, oracle, name, method="cg", **kw):
 @pytest.mark.parametrize("name", ["lap2d64", "lap3d_scaled32", "lap2d30", "knn", "powerlaw"])
 def test_cg_fp64_reaches_tolerance(mf, oracle, name):
     rep, rel, x, (rp, ci, va, b) = _solve(mf, oracle, name)
-    assert rep["status"] == 0 and rel <= TOL
+    assert rep["status"] == 0 and rel <= TOL, (rel, rep)
     assert abs(rep["rel_residual"] - rel) <= 1e-3 * TOL + 1e-15
     _, it_ref, rel_ref = oracle.cg(rp, ci, va, b, tol=TOL)
     assert abs(rep["iterations"] - it_ref) <= max(3, 0.05 * it_ref)
@@ -69,8 +69,8 @@ def test_cg_fp64_reaches_tolerance(mf, oracle, name):
 @pytest.mark.parametrize("name", ["lap2d64", "lap3d_scaled32", "powerlaw"])
 def test_cg_mixed_policies_reach_tolerance(mf, oracle, name, policy):
     rep, rel, _, _ = _solve(mf, oracle, name, policy=policy)
-    assert rep["status"] == 0 and rel <= TOL
-    assert sum(rep["prec_hist"]) > 0
+    assert rep["status"] == 0 and rel <= TOL, (rel, rep)
+    assert sum(rep["prec_hist"]) > 0, rep
 
 
 @pytest.mark.parametrize("level", [1, 2, 3])
@@ -85,22 +85,22 @@ def test_cg_fixed_low_precision_refines_to_tolerance(mf, oracle, level):
     db, dx = tm.vector(b), tm.vector()
     rep = tm.cg(db, dx, tol=TOL, policy="fixed")
     rel = oracle.rel_residual(rp, ci, va, b, dx.get()[:n])
-    assert rep["status"] == 0 and rel <= TOL
-    assert rep["refreshes"] >= 1
+    assert rep["status"] == 0 and rel <= TOL, (rel, rep)
+    assert rep["refreshes"] >= 1, rep
 
 
 @pytest.mark.parametrize("name", ["convdiff20", "lap2d30"])
 @pytest.mark.parametrize("policy", ["fp64", "rule"])
 def test_bicgstab_reaches_tolerance(mf, oracle, name, policy):
     rep, rel, x, (rp, ci, va, b) = _solve(mf, oracle, name, method="bicgstab", policy=policy)
-    assert rep["status"] == 0 and rel <= TOL
+    assert rep["status"] == 0 and rel <= TOL, (rel, rep)
 
 
 @pytest.mark.parametrize("method", ["cg_multikernel", "bicgstab_multikernel"])
 def test_multikernel_solvers(mf, oracle, method):
     name = "lap2d64" if method.startswith("cg") else "convdiff20"
     rep, rel, _, _ = _solve(mf, oracle, name, method=method)
-    assert rep["status"] == 0 and rel <= TOL
+    assert rep["status"] == 0 and rel <= TOL, (rel, rep)
 
 
 def test_solver_is_deterministic_and_takes_initial_guess(mf, oracle):
