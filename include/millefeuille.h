@@ -218,6 +218,8 @@ typedef struct mf_report {
   double time_ms;       /* kernel time (HIP events) */
   long long prec_hist[5]; /* tile-iterations per precision level */
   int grid_blocks;      /* persistent grid actually launched */
+  int policy_used;      /* MF_POLICY_* that ran (LEARNED without an installed
+                           MLP runs the rule policy) */
 } mf_report;
 
 void mf_solve_opts_default(mf_solve_opts *o);
@@ -319,10 +321,18 @@ mf_status mf_mlp_train(float *d_params, const float *d_feat,
 /* Installs trained parameters (device FP32, converted to FP16 inside) as the
  * handle's learned policy. */
 mf_status mf_policy_set_mlp(mf_tiled *t, const float *d_params);
-/* Per-tile features the policy sees (tilenum x 8 floats, device), computed
- * for a given p vector, residual norm ratio and iteration. */
+/* Per-tile features the policy sees (tilenum x 8 floats, device) for the
+ * vector d_p the SpMV multiplies (npad doubles) at relative residual
+ * res_ratio = ||r|| / ||b||: nnz, value range, FP16 and INT8 payload error,
+ * weight of the tile in its tile-row, diagonal dominance of the tile-row,
+ * local norm of the segment of d_p the tile reads, iteration progress. */
 mf_status mf_policy_features(mf_tiled *t, const double *d_p, double res_ratio,
-                             int iteration, float *d_feat, void *stream);
+                             float *d_feat, void *stream);
+/* Training labels (tilenum ints, device): for each tile the cheapest level
+ * (MF_PREC_INT8, FP16, FP32, else FP64) whose product error on d_p, computed
+ * as the SpMV computes it, is <= theta * ||A d_p||_2 / sqrt(tilenum). */
+mf_status mf_policy_labels(mf_tiled *t, const double *d_p, double theta,
+                           int *d_label, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Batched solves                                                          */

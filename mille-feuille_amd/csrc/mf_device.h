@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "mf_internal.h"
+#include "mf_mlp.h"
 
 #define MF_HIP(call)                                                         \
   do {                                                                       \
@@ -14,6 +15,12 @@
     if (e_ != hipSuccess)                                                    \
       return mf::fail(MF_ERR_HIP, std::string(#call) + ": " +               \
                                       hipGetErrorString(e_));                \
+  } while (0)
+
+#define MF_TRY(x)               \
+  do {                          \
+    mf_status s_ = (x);         \
+    if (s_ != MF_OK) return s_; \
   } while (0)
 
 namespace mf {
@@ -354,6 +361,25 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, int tr0, int tr1,
     ra = R;
     ta = tb;
   }
+}
+
+// Learned tile levels for the wave's tile-rows [tr0, tr1): the MLP runs on 16
+// tiles per MFMA batch and the levels go to ic.prec_out, where the next
+// per-tile SpMV pass of the same wave reads them (the closing workgroup
+// fence orders the stores before those loads).
+__device__ __forceinline__ void wave_infer(const TileView &A, const InferCtx &ic, int tr0, int tr1,
+                                           int lane) {
+  if (tr0 >= tr1) return;
+  const int t0 = A.tile_ptr[tr0], t1 = A.tile_ptr[tr1];
+  for (int g = t0; g < t1; g += 16) {
+    const int t = g + (lane & 15);
+    const bool valid = t < t1;
+    const int tt = valid ? t : t0;
+    const half4_t f = infer_features(ic, tt, (lane >= 16 && valid) ? A.tile_col[tt] : 0, lane, valid);
+    const float4_t z = mlp_eval16(ic.w, lane, f);
+    if (lane < 16 && valid) ic.prec_out[t] = (uint8_t)mlp_argmax(z);
+  }
+  __threadfence_block();
 }
 
 // Deterministic sum over the 64 lanes of a wave (fixed xor tree); every lane

@@ -38,8 +38,10 @@ struct DeviceLayout {
   uint8_t *prec = nullptr;     // tilenum current precision levels
   uint8_t *prec_user = nullptr;// tilenum levels set by mf_tile_set_precision
   int *diag_tile = nullptr;    // tilem: index of the diagonal tile or -1
-  // learned policy parameters (FP16 bits), MF_MLP_NPARAM
-  uint16_t *mlp16 = nullptr;
+  // learned policy: MLP in MFMA lane layout (MlpDevice) and per-tile static
+  // features (8 FP16 per tile, slots 6-7 filled per inference)
+  void *mlp = nullptr;
+  uint16_t *feat16 = nullptr;
   int has_mlp = 0;
   // stand-alone SpMV schedule: nnz-balanced tile-row ranges, one per wave
   int *spmv_tr = nullptr;

@@ -97,11 +97,6 @@ struct Buffers {
   }
 };
 
-#define MF_TRY(x)               \
-  do {                          \
-    mf_status s_ = (x);         \
-    if (s_ != MF_OK) return s_; \
-  } while (0)
 
 void fill_report(mf_report *rep, int it, double rr, double tr, double bb, double ms, int status) {
   if (!rep) return;

@@ -27,6 +27,9 @@
 
 namespace mf {
 
+// Packed MLP of a handle (mf_learned.hip), or null when none is installed.
+const MlpDevice *policy_mlp(const mf_tiled *t);
+
 constexpr int kBlock = 1024;  // 16 waves, one block per CU
 constexpr int kWavesPerBlock = kBlock / 64;
 
@@ -62,6 +65,12 @@ struct SolveArgs {
   double ratio2;              // refresh_ratio^2
   int max_refresh;
   int refresh_every;
+  // learned policy (MODE kPerTile; mlp == nullptr otherwise)
+  const MlpDevice *mlp;
+  const uint16_t *feat16;
+  float *seg2;                // per-segment ||v_seg||^2 of the multiplied vector
+  uint8_t *prec_rw;           // tile levels written by the inference
+  int nseg;
 };
 
 // Reference rule (src/main-cg-mixed.cu:1326-1363): level from the absolute
@@ -79,6 +88,26 @@ __device__ __forceinline__ double group16_max(double v) {
 #pragma unroll
   for (int m = 8; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m));
   return v;
+}
+
+// Sum over the 16 lanes of each group.
+__device__ __forceinline__ double group16_sum(double v) {
+#pragma unroll
+  for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+// Inference context of the learned policy for a vector of squared norm v2.
+__device__ __forceinline__ InferCtx infer_ctx(const SolveArgs &a, double v2, double rr, double bb) {
+  InferCtx c;
+  c.w = a.mlp;
+  c.feat16 = a.feat16;
+  c.seg2 = a.seg2;
+  c.prec_out = a.prec_rw;
+  c.seg_scale = v2 > 0.0 ? (float)(sqrt((double)a.nseg) / sqrt(v2)) : 0.0f;
+  const double ratio = bb > 0.0 ? sqrt(rr / bb) : 1.0;
+  c.progress = (float)(log10(fmax(ratio, 1e-16)) / 16.0);
+  return c;
 }
 
 __device__ __forceinline__ void flush_hist(const PrecCount &c, unsigned long long *hist,
@@ -139,6 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   const int tr0 = a.wave_tr[wid], tr1 = a.wave_tr[wid + 1];
   const int row0 = tr0 * 16, row1 = tr1 * 16;
   constexpr bool mixed = MODE != kAll64;
+  const bool learned = MODE == kPerTile && a.mlp != nullptr;
   unsigned epoch = 0;
   int slot = 0;
   PrecCount cnt;
@@ -154,6 +184,15 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       red[0] += rv * rv;
       red[1] += bv * bv;
     });
+  if (learned) {
+    // segment norms of p = r0 for the first inference
+    for (int base = row0; base < row1; base += 64) {
+      const int i = base + lane;
+      const double pv = i < row1 ? a.p[i] : 0.0;
+      const double d2 = group16_sum(pv * pv);
+      if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d2;
+    }
+  }
   grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
   double rr = red[0];
   const double bb = red[1];
@@ -162,6 +201,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0;
   bool restart = false;
   bool done = bb == 0.0 || (a.fixed_iters <= 0 && rr <= stop2);
+  bool infer = learned;  // learned policy: new levels in the next SpMV
+  double pp_last = rr;   // ||p||^2 at the last inference point
 
   while (!done) {
     if (restart) {
@@ -178,6 +219,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
     ++it;
     // q = A p, fused with p.q for the wave's rows.
     double pq[1] = {0.0};
+    if (infer && !force64) wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
+    infer = false;
     wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
       [&](const int i, const double qv) {
         a.q[i] = qv;
@@ -235,6 +278,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
     rr = rr_new;
     // p = r + beta p; the rule policy looks at |p_new - p_old| per segment.
     const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
+    const bool lupd = learned && !force64 && (it % a.check_interval) == 0;
+    double pp[1] = {0.0};
     for (int base = row0; base < row1; base += 64) {
       const int i = base + lane;
       double d = 0.0;
@@ -243,14 +288,25 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
         const double pv = a.r[i] + beta * po;
         a.p[i] = pv;
         if (mixed) a.p32[i] = (float)pv;
-        d = fabs(pv - po);
+        d = lupd ? pv * pv : fabs(pv - po);
       }
       if (upd) {
         d = group16_max(d);
         if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
       }
+      if (lupd) {
+        pp[0] += d;
+        d = group16_sum(d);
+        if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d;
+      }
     }
-    grid_sync(a.gs, epoch, &lds_i);
+    if (lupd) {
+      grid_sum<1>(pp, a.partials, a.gs, epoch, slot, lds, &lds_i);
+      pp_last = pp[0];
+      infer = true;
+    } else {
+      grid_sync(a.gs, epoch, &lds_i);
+    }
   }
   if (mixed) flush_hist(cnt, a.hist, lds);
   // Final FP64 residual for the report.
@@ -285,6 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   const int tr0 = a.wave_tr[wid], tr1 = a.wave_tr[wid + 1];
   const int row0 = tr0 * 16, row1 = tr1 * 16;
   constexpr bool mixed = MODE != kAll64;
+  const bool learned = MODE == kPerTile && a.mlp != nullptr;
   unsigned epoch = 0;
   int slot = 0;
   PrecCount cnt;
@@ -301,6 +358,15 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
       red[0] += rv * rv;
       red[1] += bv * bv;
     });
+  if (learned) {
+    // segment norms of p = r0 for the first inference
+    for (int base = row0; base < row1; base += 64) {
+      const int i = base + lane;
+      const double pv = i < row1 ? a.p[i] : 0.0;
+      const double d2 = group16_sum(pv * pv);
+      if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d2;
+    }
+  }
   grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
   double rho = red[0], rr = red[0];
   const double bb = red[1];
@@ -309,6 +375,8 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0, restarts = 0;
   bool restart = false;
   bool done = bb == 0.0 || (a.fixed_iters <= 0 && rr <= stop2);
+  bool infer = learned;  // learned policy: new levels in the next v = A p
+  double pp_last = rr;   // ||p||^2 at the last inference point
 
   while (!done) {
     if (restart) {
@@ -325,6 +393,8 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     ++it;
     // v = A p, v.rh
     double vr[1] = {0.0};
+    if (infer && !force64) wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
+    infer = false;
     wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
       [&](const int i, const double vv) {
         v[i] = vv;
@@ -399,6 +469,8 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
           force64 = 1;
           restart = true;
         }
+        // A recursive residual far below the true one makes beta meaningless.
+        if (tr[0] > 4.0 * rr_new) restart = true;
         rr_new = tr[0];
         rho_new = tr[1];
         rr_ref = tr[0];
@@ -412,6 +484,8 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     rr = rr_new;
     // p = r + beta (p - omega v)
     const bool updp = MODE == kPerSeg && (it % a.check_interval) == 0;
+    const bool lupd = learned && !force64 && (it % a.check_interval) == 0;
+    double pp[1] = {0.0};
     for (int base = row0; base < row1; base += 64) {
       const int i = base + lane;
       double d = 0.0;
@@ -420,14 +494,25 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
         const double pv = a.r[i] + beta * (po - omega * v[i]);
         a.p[i] = pv;
         if (mixed) a.p32[i] = (float)pv;
-        d = fabs(pv - po);
+        d = lupd ? pv * pv : fabs(pv - po);
       }
       if (updp) {
         d = group16_max(d);
         if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
       }
+      if (lupd) {
+        pp[0] += d;
+        d = group16_sum(d);
+        if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d;
+      }
     }
-    grid_sync(a.gs, epoch, &lds_i);
+    if (lupd) {
+      grid_sum<1>(pp, a.partials, a.gs, epoch, slot, lds, &lds_i);
+      pp_last = pp[0];
+      infer = true;
+    } else {
+      grid_sync(a.gs, epoch, &lds_i);
+    }
   }
   if (mixed) flush_hist(cnt, a.hist, lds);
   double fin[1] = {true_residual_rows(a, tr0, tr1, lane, nullptr, wl)};
@@ -493,6 +578,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
   double *X = xbuf + wv * K * 64;  // x of the wave's rows
   WaveLds &L = wl[wv];
   constexpr bool mixed = MODE != kAll64;
+  const bool learned = MODE == kPerTile && a.mlp != nullptr;
   unsigned epoch = 0;
   int slot = 0;
   PrecCount cnt;
@@ -507,6 +593,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int li = ln0 + 64 * k;
+    double d2 = 0.0;
     p[k] = s[k] = 0.0;
     if (li < nrows) {
       const int i = row0 + li;
@@ -517,6 +604,11 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
       red[1] += bv * bv;
       a.r[i] = rv;
       if (mixed) a.p32[i] = (float)rv;
+      d2 = rv * rv;
+    }
+    if (learned) {
+      d2 = group16_sum(d2);
+      if ((lane & 15) == 0 && li < nrows) a.seg2[(row0 + li) >> 4] = (float)d2;
     }
   }
   grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
@@ -529,10 +621,15 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
   bool done = bb == 0.0 || (a.fixed_iters <= 0 && rr <= stop2);
 
   bool pending = false;  // recompute the FP64 residual before the next step
+  bool infer = learned;  // learned policy: new levels in the next SpMV
   while (!done) {
     double gamma = 0.0, delta = 0.0;
     if (!pending) {
-      // w = A r (r published by every wave before the last sync)
+      // w = A r (r published by every wave before the last sync); the
+      // learned policy re-evaluates the tile levels every check interval
+      // from the segment norms of r written before that sync.
+      if (infer && !force64) wave_infer(a.A, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
+      infer = false;
       wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.r, a.p32, a.seg_p, force64, L, lane, &cnt, epi_w);
       __builtin_amdgcn_wave_barrier();
       double gd[2] = {0.0, 0.0};
@@ -643,6 +740,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
     alpha_old = alpha;
     ++it;
     const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
+    const bool lupd = learned && !force64 && (it % a.check_interval) == 0;
     const int ln6 = opaque(lane);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -658,12 +756,18 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
         d = fabs(rn - rv);
         a.r[i] = rn;
         if (mixed) a.p32[i] = (float)rn;
+        if (lupd) d = rn * rn;
       }
       if (upd) {
         d = group16_max(d);
         if ((lane & 15) == 0 && li < nrows) a.seg_p[(row0 + li) >> 4] = (uint8_t)rule_level(d);
       }
+      if (lupd) {
+        d = group16_sum(d);
+        if ((lane & 15) == 0 && li < nrows) a.seg2[(row0 + li) >> 4] = (float)d;
+      }
     }
+    infer = lupd;
     grid_sync(a.gs, epoch, &lds_i);
     if (a.fixed_iters > 0 && it >= a.fixed_iters) break;
   }
@@ -721,6 +825,7 @@ struct Workspace {
   double *r, *p, *q, *rh, *s, *tv;
   float *p32, *s32;
   uint8_t *seg_p, *seg_s;
+  float *seg2;
   double *partials;
   GridSync *gs;
   double *stats;
@@ -732,7 +837,7 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 // Carves the handle's device workspace (grown on demand).
 mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
-  size_t need = 6 * align256(n * 8) + 2 * align256(n * 4) + 2 * align256(segs) +
+  size_t need = 6 * align256(n * 8) + 2 * align256(n * 4) + 2 * align256(segs) + align256(segs * 4) +
                 align256(2 * kMaxGridSum * (size_t)G * 8) + align256(sizeof(GridSync)) + align256(kStCount * 8) +
                 align256(5 * 8);
   if (t->work_bytes < need) {
@@ -754,6 +859,7 @@ mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   w.s32 = (float *)take(n * 4);
   w.seg_p = (uint8_t *)take(segs);
   w.seg_s = (uint8_t *)take(segs);
+  w.seg2 = (float *)take(segs * 4);
   w.partials = (double *)take(2 * kMaxGridSum * (size_t)G * 8);
   w.gs = (GridSync *)take(sizeof(GridSync));
   w.stats = (double *)take(kStCount * 8);
@@ -823,8 +929,8 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   if (!(opt.tol > 0.0)) opt.tol = 1e-10;
   if (opt.policy < MF_POLICY_FP64 || opt.policy > MF_POLICY_ERRBOUND)
     return fail(MF_ERR_ARG, "solve: unknown policy");
-  if (opt.policy == MF_POLICY_LEARNED && !t->dev.has_mlp)
-    return fail(MF_ERR_ARG, "solve: learned policy requested but no MLP installed (mf_policy_set_mlp)");
+  // Without an installed MLP the learned policy runs the reference's rule.
+  if (opt.policy == MF_POLICY_LEARNED && !t->dev.has_mlp) opt.policy = MF_POLICY_RULE;
   MF_HIP(hipSetDevice(t->dev.device));
   hipStream_t st = (hipStream_t)opt.stream;
   const int mode = spmv_mode_for(opt.policy);
@@ -903,6 +1009,13 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   a.ratio2 = opt.refresh_ratio * opt.refresh_ratio;
   a.max_refresh = opt.max_refresh;
   a.refresh_every = opt.refresh_every;
+  const bool learned = opt.policy == MF_POLICY_LEARNED;
+  a.mlp = learned ? policy_mlp(t) : nullptr;
+  a.feat16 = t->dev.feat16;
+  a.seg2 = w.seg2;
+  a.prec_rw = t->dev.prec;
+  a.nseg = (int)segs;
+  if (learned) MF_HIP(hipMemsetAsync(w.seg2, 0, segs * 4, st));
 
   hipEvent_t e0, e1;
   MF_HIP(hipEventCreate(&e0));
@@ -950,6 +1063,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   r.time_ms = ms;
   for (int i = 0; i < 5; ++i) r.prec_hist[i] = (long long)hist[i];
   r.grid_blocks = G;
+  r.policy_used = opt.policy;
   int ret = status;
   if (ret == MF_OK && opt.fixed_iters <= 0 && !(r.rel_residual <= opt.tol)) ret = MF_NOT_CONVERGED;
   r.status = ret;

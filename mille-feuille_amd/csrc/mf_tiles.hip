@@ -94,11 +94,6 @@ static mf_status dalloc(T **p, size_t n) {
   return MF_OK;
 }
 
-#define MF_TRY(x)                 \
-  do {                            \
-    mf_status s_ = (x);           \
-    if (s_ != MF_OK) return s_;   \
-  } while (0)
 
 mf_status upload_tiles(mf_tiled *t, int device) {
   DeviceLayout &d = t->dev;
@@ -149,7 +144,6 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_TRY(dalloc(&d.prec, T));
   MF_TRY(dalloc(&d.prec_user, T));
   MF_TRY(dalloc(&d.diag_tile, TM));
-  MF_TRY(dalloc(&d.mlp16, MF_MLP_NPARAM));
   MF_HIP(hipMemcpy(d.tile_ptr, t->tile_ptr.data(), sizeof(int) * (TM + 1), hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.tile_col, t->tile_columnidx.data(), sizeof(int) * T, hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.tile_off, t->csr_offset.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice));
@@ -181,7 +175,7 @@ void free_device(mf_tiled *t) {
   if (d.device >= 0) (void)hipSetDevice(d.device);
   void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.spmv_tr,
                   d.v64,      d.v32,      d.v16,      d.v8,       d.s16,    d.s8,
-                  d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp16,
+                  d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp, d.feat16,
                   t->sched.wave_tr, t->work};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);

@@ -53,7 +53,8 @@ class MfReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("refreshes", C.c_int), ("fell_back", C.c_int),
                 ("status", C.c_int), ("rel_residual", C.c_double),
                 ("rel_residual_recursive", C.c_double), ("time_ms", C.c_double),
-                ("prec_hist", C.c_longlong * 5), ("grid_blocks", C.c_int)]
+                ("prec_hist", C.c_longlong * 5), ("grid_blocks", C.c_int),
+                ("policy_used", C.c_int)]
 
     def as_dict(self):
         return {"iterations": self.iterations, "refreshes": self.refreshes,
@@ -61,7 +62,7 @@ class MfReport(C.Structure):
                 "rel_residual": self.rel_residual,
                 "rel_residual_recursive": self.rel_residual_recursive,
                 "time_ms": self.time_ms, "prec_hist": list(self.prec_hist),
-                "grid_blocks": self.grid_blocks}
+                "grid_blocks": self.grid_blocks, "policy_used": self.policy_used}
 
 
 _vp = C.c_void_p
@@ -120,7 +121,8 @@ SIGNATURES = {
     "mf_mlp_train": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint,
                                C.POINTER(C.c_float), _vp]),
     "mf_policy_set_mlp": (C.c_int, [_vp, _vp]),
-    "mf_policy_features": (C.c_int, [_vp, _vp, C.c_double, C.c_int, _vp, _vp]),
+    "mf_policy_features": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp]),
+    "mf_policy_labels": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp]),
     "mf_batch_cg_solve": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.POINTER(_vp),
                                     C.POINTER(_vp), C.POINTER(MfSolveOpts), _ip,
                                     C.POINTER(C.c_double), C.POINTER(C.c_double),

@@ -55,6 +55,7 @@ for name, res, args in [
     ("or_quant_int8", C.c_float, [_v, _I, _v]),
     ("or_rule_policy", None, [_I, _v, _v, _v]),
     ("or_mlp_forward", None, [_I, _v, _v, _v, _v, _v, _v, _v, _v]),
+    ("or_mlp_forward16", None, [_I, _v, _v, _v, _v, _v, _v, _v, _v]),
     ("or_mlp_loss", C.c_double, [_I, _v, _v, _v, _v, _v, _v, _v, _v]),
     ("or_mlp_grad", C.c_double, [_I, _v, _v] + [_v] * 6 + [_v] * 6),
 ]:
@@ -278,6 +279,18 @@ def mlp_forward(params, feat):
     n = len(f) // 8
     z = np.zeros(max(n, 1) * 4)
     _o.or_mlp_forward(n, _p(f), *(_p(a) for a in w), _p(z))
+    return z[: 4 * n].reshape(n, 4)
+
+
+def mlp_forward16(params, feat):
+    """Forward pass with the device's FP16 rounding of parameters, features
+    and hidden activations (the parameters are rounded here)."""
+    p16 = np.asarray(params, dtype=np.float32).astype(np.float16).astype(np.float64)
+    w = mlp_split(p16)
+    f = _f64(feat).ravel()
+    n = len(f) // 8
+    z = np.zeros(max(n, 1) * 4)
+    _o.or_mlp_forward16(n, _p(f), *(_p(a) for a in w), _p(z))
     return z[: 4 * n].reshape(n, 4)
 
 

@@ -862,6 +862,36 @@ void or_mlp_forward(int nrows, const double *feat, const double *w1,
     mlp_row(feat + r * NI, w1, b1, w2, b2, w3, b3, a1, a2, logits + r * NO);
 }
 
+/* The device evaluation's numerics (MFMA f16 inputs, FP32 accumulation):
+ * parameters are given already rounded to FP16; features and both hidden
+ * activations are rounded to FP16 (round to nearest even, through FP32)
+ * before they enter the next layer; sums here are FP64, the device's FP32. */
+static double h16(double v) { return (double)or_float_from_half(or_half_from_float((float)v)); }
+
+void or_mlp_forward16(int nrows, const double *feat, const double *w1,
+                      const double *b1, const double *w2, const double *b2,
+                      const double *w3, const double *b3, double *logits) {
+  double f[NI], a1[H1], a2[H2];
+  for (int r = 0; r < nrows; ++r) {
+    for (int i = 0; i < NI; ++i) f[i] = h16(feat[r * NI + i]);
+    for (int o = 0; o < H1; ++o) {
+      double s = b1[o];
+      for (int i = 0; i < NI; ++i) s += w1[o * NI + i] * f[i];
+      a1[o] = h16(s > 0.0 ? s : 0.0);
+    }
+    for (int o = 0; o < H2; ++o) {
+      double s = b2[o];
+      for (int i = 0; i < H1; ++i) s += w2[o * H1 + i] * a1[i];
+      a2[o] = h16(s > 0.0 ? s : 0.0);
+    }
+    for (int o = 0; o < NO; ++o) {
+      double s = b3[o];
+      for (int i = 0; i < H2; ++i) s += w3[o * H2 + i] * a2[i];
+      logits[r * NO + o] = s;
+    }
+  }
+}
+
 static double row_ce(const double *z, int label, double *dz) {
   double mx = z[0];
   for (int o = 1; o < NO; ++o) if (z[o] > mx) mx = z[o];

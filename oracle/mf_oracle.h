@@ -42,6 +42,10 @@
  *                                  max|a|/2^e <= 127, q = rint(a/scale).
  *   or_rule_policy                 hand-computed segments at each threshold.
  *   or_mlp_*                       central finite differences of or_mlp_loss.
+ *   or_mlp_forward16               equals or_mlp_forward wherever every
+ *                                  feature and activation is FP16-representable
+ *                                  (small dyadic weights), and differs from it
+ *                                  by the FP16 rounding otherwise.
  */
 #ifndef MF_ORACLE_H
 #define MF_ORACLE_H
@@ -134,6 +138,11 @@ void or_rule_policy(int nseg, const double *p_new, const double *p_old,
 void or_mlp_forward(int nrows, const double *feat, const double *w1,
                     const double *b1, const double *w2, const double *b2,
                     const double *w3, const double *b3, double *logits);
+/* Same network with the device's FP16 rounding of features and hidden
+ * activations (parameters passed already FP16-representable). */
+void or_mlp_forward16(int nrows, const double *feat, const double *w1,
+                      const double *b1, const double *w2, const double *b2,
+                      const double *w3, const double *b3, double *logits);
 double or_mlp_loss(int nrows, const double *feat, const int *label,
                    const double *w1, const double *b1, const double *w2,
                    const double *b2, const double *w3, const double *b3);

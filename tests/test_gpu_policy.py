@@ -1,0 +1,269 @@
+"""Learned tile-precision policy on the GPU: MLP kernels against the CPU
+oracle, features and labels against numpy restatements, training, and the
+learned policy inside the single-kernel solvers.
+
+Tolerances:
+  * forward: the device evaluates with FP16 features/activations and FP32
+    accumulation (MFMA); the oracle's forward16 rounds the same values in
+    FP64 arithmetic, so a hidden activation may differ by one FP16 ulp:
+    |z_gpu - z_oracle| <= 5e-3 (1 + |z|), argmax agreement >= 99%.
+  * gradient: FP32 sums of 1476 parameters over the batch against the
+    oracle's FP64 at the same FP16-rounded weights: relative 2-norm error
+    <= 1e-4, loss within 1e-5 relative.
+  * features: stored as FP16 (slots 0-5) and rounded to FP16 (6-7):
+    absolute error <= 2e-3.
+  * labels: same arithmetic as the SpMV (FP32 products of the payloads),
+    restated in numpy float32; >= 99.5% identical (ties at the threshold may
+    round either way).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _entries(tm):
+    """Row, column and value of every stored entry, in payload order."""
+    a = tm.arrays()
+    off = a["csr_offset"]
+    nz = np.diff(off)
+    val = a["Blockcsr_Val"]
+    rows = np.zeros(len(val), np.int64)
+    cols = np.zeros(len(val), np.int64)
+    for bi in range(tm.tilem):
+        h = min(16, tm.rowA - 16 * bi)
+        for t in range(a["tile_ptr"][bi], a["tile_ptr"][bi + 1]):
+            rp = a["Blockcsr_Ptr"][a["csrptr_offset"][t]: a["csrptr_offset"][t] + h].astype(np.int64)
+            ends = np.append(rp[1:], nz[t])
+            for r in range(h):
+                rows[off[t] + rp[r]: off[t] + ends[r]] = 16 * bi + r
+            cols[off[t]: off[t + 1]] = 16 * a["tile_columnidx"][t] + a["Tile_csr_Col"][off[t]: off[t + 1]]
+    tile_of = np.repeat(np.arange(tm.tilenum), nz)
+    return a, rows, cols, val, tile_of
+
+
+def _system(name):
+    from millefeuille import gen
+    if name == "lap3d_scaled16":
+        rp, ci, va = gen.laplace3d_scaled(16, seed=2)
+        return rp, ci, va, gen.uniform_vector(len(rp) - 1, 2)
+    if name == "lap3d_scaled32":
+        rp, ci, va = gen.laplace3d_scaled(32, seed=2)
+        return rp, ci, va, gen.uniform_vector(len(rp) - 1, 2)
+    if name == "knn":
+        rp, ci, va = gen.knn_laplacian(3000, 8, seed=4)
+        return rp, ci, va, gen.uniform_vector(len(rp) - 1, 4)
+    if name == "powerlaw":
+        rp, ci, va = gen.powerlaw_laplacian(20000, 300000, seed=5)
+        return rp, ci, va, gen.uniform_vector(len(rp) - 1, 5)
+    if name == "convdiff20":
+        rp, ci, va = gen.convdiff3d(20)
+        xs = gen.uniform_vector(len(rp) - 1, 3)
+        return rp, ci, va, gen.csr_matvec(rp, ci, va, xs)
+    raise KeyError(name)
+
+
+@pytest.fixture(scope="module")
+def policy():
+    from millefeuille import policy as pol
+    return pol
+
+
+@pytest.fixture(scope="module")
+def small_params(policy):
+    """A policy trained on the GPU on traces of shrunken training workloads."""
+    params, loss, acc, hist = policy.train_default(scale=0.25, epochs=20, seed=1)
+    print("small policy: loss", loss, "accuracy", acc, "labels", hist, flush=True)
+    return params
+
+
+def test_mlp_forward_matches_oracle(oracle, policy):
+    rng = np.random.default_rng(21)
+    params = (rng.standard_normal(1476) * 0.4).astype(np.float32)
+    feat = rng.standard_normal((1003, 8)).astype(np.float32)  # not a multiple of 16
+    z = policy.forward(params, feat)
+    zo = oracle.mlp_forward16(params, feat)
+    assert np.all(np.abs(z - zo) <= 5e-3 * (1.0 + np.abs(zo))), np.max(np.abs(z - zo))
+    assert np.mean(np.argmax(z, 1) == np.argmax(zo, 1)) >= 0.99
+
+
+def test_mlp_grad_matches_oracle(oracle, policy):
+    rng = np.random.default_rng(22)
+    params = (rng.standard_normal(1476) * 0.3).astype(np.float32)
+    feat = rng.standard_normal((777, 8)).astype(np.float32)
+    lab = rng.integers(0, 4, 777).astype(np.int32)
+    loss, g = policy.grad(params, feat, lab)
+    p16 = params.astype(np.float16).astype(np.float64)
+    lo, go = oracle.mlp_grad(p16, feat.astype(np.float64), lab)
+    assert abs(loss - lo) <= 1e-5 * abs(lo)
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+
+
+def test_mlp_train_fits_separable_data(policy):
+    rng = np.random.default_rng(23)
+    feat = rng.standard_normal((20000, 8)).astype(np.float32)
+    M = rng.standard_normal((8, 4))
+    lab = np.argmax(feat @ M, axis=1).astype(np.int32)
+    p0 = policy.init_params(3)
+    l0, _ = policy.grad(p0, feat, lab)
+    params, loss = policy.train(feat, lab, params=p0, epochs=25, batch=1024, lr=3e-3, seed=3)
+    acc = np.mean(np.argmax(policy.forward(params, feat), 1) == lab)
+    assert loss < 0.5 * l0 and acc >= 0.9, (l0, loss, acc)
+
+
+def test_policy_features_match_restatement(mf, policy):
+    rp, ci, va, b = _system("lap3d_scaled16")
+    tm = mf.TiledMatrix(rp, ci, va)
+    a, rows, cols, val, tile_of = _entries(tm)
+    T = tm.tilenum
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(tm.npad)
+    v[tm.rowA:] = 0.0
+    v[: 16 * 40] *= 1e-4  # give the local-norm feature some spread
+    f = policy.features(tm, tm.vector(v), 3e-4)
+    av = np.abs(val)
+    nz = np.diff(a["csr_offset"])
+    amax = np.maximum.reduceat(av, a["csr_offset"][:-1])
+    amin = np.minimum.reduceat(np.where(av > 0, av, np.inf), a["csr_offset"][:-1])
+    q = tm.quant_error()
+    tsum = np.add.reduceat(av, a["csr_offset"][:-1])
+    trow = np.repeat(np.arange(tm.tilem), np.diff(a["tile_ptr"]))
+    rowsum = np.bincount(trow, tsum, minlength=tm.tilem)
+    diag = np.bincount(rows, np.where(rows == cols, av, 0.0), minlength=16 * tm.tilem)
+    off = np.bincount(rows, np.where(rows != cols, av, 0.0), minlength=16 * tm.tilem)
+    has = np.bincount(rows, minlength=16 * tm.tilem) > 0
+    with np.errstate(divide="ignore"):
+        dom = np.where(off > 0, diag / np.where(off > 0, off, 1), np.inf)
+    dom = np.where(has, dom, np.inf).reshape(tm.tilem, 16).min(1)
+    fd = np.where(np.isinf(dom), 1.0, np.where(dom > 0, np.log2(np.where(dom > 0, dom, 1)) / 8, -1.0))
+    nseg = max(tm.tilem, tm.tilen)
+    seg = np.sqrt((v[: 16 * nseg].reshape(nseg, 16) ** 2).sum(1))
+    ref = np.zeros((T, 8))
+    ref[:, 0] = nz / 256
+    ref[:, 1] = np.log2(amax / amin) / 24
+    ref[:, 2] = np.log2(q[1] / amax + 2.0 ** -40) / 40
+    ref[:, 3] = np.log2(q[2] / amax + 2.0 ** -40) / 40
+    ref[:, 4] = tsum / rowsum[trow]
+    ref[:, 5] = np.clip(fd, -1, 1)[trow]
+    ref[:, 6] = np.log2(seg[a["tile_columnidx"]] * np.sqrt(nseg) / np.linalg.norm(v) + 2.0 ** -40) / 40
+    ref[:, 7] = np.log10(3e-4) / 16
+    np.testing.assert_allclose(f, ref, atol=2e-3, rtol=0)
+
+
+def test_policy_labels_match_restatement(mf, policy):
+    rp, ci, va, b = _system("lap3d_scaled16")
+    tm = mf.TiledMatrix(rp, ci, va)
+    a, rows, cols, val, tile_of = _entries(tm)
+    s16, s8 = tm.scales()
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal(tm.npad)
+    v[tm.rowA:] = 0.0
+    v[::7] *= 1e-5
+    f32 = np.float32
+    x = v[cols]
+    x32 = x.astype(f32)
+    exact = val * x
+    v32 = val.astype(f32)
+    inv16 = (1.0 / s16[tile_of]).astype(f32)
+    h = (v32 * inv16).astype(np.float16)
+    p32 = (v32 * x32).astype(np.float64)
+    p16 = ((h.astype(f32) * x32) * s16[tile_of]).astype(np.float64)
+    q8 = np.clip(np.rint(val / s8[tile_of]), -127, 127).astype(f32)
+    p8 = ((q8 * x32) * s8[tile_of]).astype(np.float64)
+    key = tile_of * 16 + (rows % 16)
+    T = tm.tilenum
+    errs = []
+    for p in (p32, p16, p8):
+        row_err = np.bincount(key, p - exact, minlength=16 * T)
+        errs.append((row_err.reshape(T, 16) ** 2).sum(1))
+    av = np.bincount(rows, exact, minlength=16 * tm.tilem)
+    for theta in (1e-3, 1e-5, 1e-7):
+        thr2 = (theta * np.linalg.norm(av) / np.sqrt(T)) ** 2
+        ref = np.zeros(T, np.int64)
+        for lv, err in zip((1, 2, 3), errs):
+            ref = np.where(err <= thr2, lv, ref)
+        got = policy.labels(tm, tm.vector(v), theta)
+        assert np.mean(got == ref) >= 0.995, (theta, np.mean(got == ref))
+
+
+@pytest.mark.parametrize("name", ["lap3d_scaled32", "knn", "powerlaw"])
+@pytest.mark.parametrize("resident", [True, False])
+def test_learned_cg_reaches_tolerance(mf, oracle, policy, small_params, name, resident, monkeypatch):
+    if not resident:
+        monkeypatch.setenv("MF_NO_RESIDENT", "1")
+    print(f"case learned cg {name} resident={resident}", flush=True)
+    rp, ci, va, b = _system(name)
+    n = len(rp) - 1
+    tm = mf.TiledMatrix(rp, ci, va)
+    policy.install(tm, small_params)
+    db, dx = tm.vector(b), tm.vector()
+    rep = tm.cg(db, dx, tol=TOL, policy="learned")
+    rel = oracle.rel_residual(rp, ci, va, b, dx.get()[:n])
+    assert rep["status"] == 0 and rel <= TOL, (rel, rep)
+    assert rep["policy_used"] == mf.POLICY_LEARNED
+    assert sum(rep["prec_hist"][1:4]) > 0, rep
+
+
+def test_learned_bicgstab_reaches_tolerance(mf, oracle, policy, small_params):
+    rp, ci, va, b = _system("convdiff20")
+    n = len(rp) - 1
+    tm = mf.TiledMatrix(rp, ci, va)
+    policy.install(tm, small_params)
+    db, dx = tm.vector(b), tm.vector()
+    rep = tm.bicgstab(db, dx, tol=TOL, policy="learned")
+    rel = oracle.rel_residual(rp, ci, va, b, dx.get()[:n])
+    assert rep["status"] == 0 and rel <= TOL, (rel, rep)
+    assert rep["policy_used"] == mf.POLICY_LEARNED
+
+
+def test_learned_without_mlp_runs_rule(mf):
+    rp, ci, va, b = _system("lap3d_scaled16")
+    tm = mf.TiledMatrix(rp, ci, va)
+    db, dx = tm.vector(b), tm.vector()
+    rep = tm.cg(db, dx, tol=TOL, policy="learned")
+    assert rep["status"] == 0 and rep["policy_used"] == mf.POLICY_RULE
+
+
+def _payload_bytes(tm, hist):
+    """Payload bytes implied by the tile-visit histogram (average tile)."""
+    avg = tm.csrsize / max(tm.tilenum, 1)
+    size = np.array([9, 5, 3, 2, 0])
+    return float(np.dot(hist, size) * avg)
+
+
+def test_learned_policy_gains_over_rule(mf, oracle, policy, small_params):
+    """What the trained network gains over the reference's rule: both runs
+    reach the FP64-checked 1e-10; the rule's absolute thresholds send tiles
+    to SKIP, lose definiteness and fall back to FP64, while the learned
+    policy keeps most tiles at INT8/FP32 to the end, moving at most 60% of
+    the FP64 payload bytes per SpMV and fewer bytes in total than the rule."""
+    rp, ci, va, b = _system("lap3d_scaled32")
+    n = len(rp) - 1
+    tm = mf.TiledMatrix(rp, ci, va)
+    policy.install(tm, small_params)
+    db = tm.vector(b)
+    out = {}
+    for pol in ("fp64", "rule", "learned"):
+        dx = tm.vector()
+        rep = tm.cg(db, dx, tol=TOL, policy=pol)
+        rel = oracle.rel_residual(rp, ci, va, b, dx.get()[:n])
+        assert rep["status"] == 0 and rel <= TOL, (pol, rel, rep)
+        out[pol] = rep
+        print(pol, rep["iterations"], rep["refreshes"], rep["fell_back"], f"{rep['time_ms']:.2f} ms",
+              rep["prec_hist"], flush=True)
+    le, ru, f64 = out["learned"], out["rule"], out["fp64"]
+    spmv_l = le["iterations"] + le["refreshes"]
+    fp64_bytes_per_spmv = 9.0 * tm.csrsize
+    assert le["fell_back"] == 0
+    assert _payload_bytes(tm, le["prec_hist"]) / spmv_l <= 0.6 * fp64_bytes_per_spmv
+    # total payload bytes: the histogram covers every iteration's SpMV (FP64
+    # visits after a fallback included); each FP64 recomputation adds one
+    # FP64 SpMV
+    rule_total = _payload_bytes(tm, ru["prec_hist"]) + fp64_bytes_per_spmv * ru["refreshes"]
+    learned_total = _payload_bytes(tm, le["prec_hist"]) + fp64_bytes_per_spmv * le["refreshes"]
+    assert learned_total < rule_total, (learned_total, rule_total)
+    assert le["iterations"] <= 1.3 * f64["iterations"] + 10

@@ -324,3 +324,38 @@ def test_mlp_gradient_finite_differences(oracle):
     a2 = np.maximum(a1 @ w2.reshape(32, 32).T + b2, 0)
     z = a2 @ w3.reshape(4, 32).T + b3
     np.testing.assert_allclose(oracle.mlp_forward(params, feat), z, rtol=1e-12, atol=1e-12)
+
+
+def test_mlp_forward16_rounding(oracle):
+    """or_mlp_forward16 is or_mlp_forward with FP16 rounding of features and
+    activations: identical when every value is FP16-representable, close to
+    a numpy float16 restatement otherwise."""
+    rng = np.random.default_rng(12)
+    # dyadic network whose features and activations are all exact in FP16
+    p = np.zeros(1476)
+    w1, b1, w2, b2, w3, b3 = (slice(0, 256), slice(256, 288), slice(288, 1312),
+                              slice(1312, 1344), slice(1344, 1472), slice(1472, 1476))
+    p[w1] = rng.integers(-1, 2, 256) * 0.5
+    p[b1] = rng.integers(-2, 3, 32) * 0.25
+    p[w2] = rng.integers(-1, 2, 1024) * 0.25
+    p[b2] = rng.integers(-2, 3, 32) * 0.25
+    p[w3] = rng.integers(-4, 5, 128) * 0.125
+    p[b3] = rng.integers(-4, 5, 4) * 0.125
+    feat = rng.integers(-2, 3, (64, 8)) * 0.5
+    np.testing.assert_array_equal(oracle.mlp_forward16(p, feat), oracle.mlp_forward(p, feat))
+    # general weights: numpy restatement with float16 rounding between layers
+    p = rng.standard_normal(1476) * 0.4
+    feat = rng.standard_normal((64, 8))
+    q = p.astype(np.float32).astype(np.float16).astype(np.float64)
+    h = lambda v: v.astype(np.float32).astype(np.float16).astype(np.float64)
+    f = h(feat)
+    a1 = h(np.maximum(f @ q[w1].reshape(32, 8).T + q[b1], 0))
+    a2 = h(np.maximum(a1 @ q[w2].reshape(32, 32).T + q[b2], 0))
+    z = a2 @ q[w3].reshape(4, 32).T + q[b3]
+    z16 = oracle.mlp_forward16(p, feat)
+    # identical up to the (rare) FP16 rounding flip caused by a different
+    # FP64 summation order
+    close = np.isclose(z16, z, rtol=1e-12, atol=1e-12).all(axis=1)
+    assert close.mean() >= 0.95
+    np.testing.assert_allclose(z16, z, rtol=1e-2, atol=1e-2)
+    assert not np.allclose(z16, oracle.mlp_forward(p, feat), rtol=1e-9, atol=1e-9)
