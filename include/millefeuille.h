@@ -167,6 +167,16 @@ mf_status mf_tile_get_quant_error(const mf_tiled *t, double *err);
 mf_status mf_spmv(mf_tiled *t, const double *d_x, const float *d_x32,
                   double *d_y, int mode, void *stream);
 
+/* Stand-alone SpMV kernel of a handle: MF_SPMV_STREAM (nnz-balanced tile
+ * ranges, LDS row accumulators; any sparsity), MF_SPMV_ROWGROUP (one row per
+ * lane, four tile-rows per wave; regular sparsity) or MF_SPMV_AUTO (row-group
+ * when no tile-row holds more than 4x the mean tile count + 8; the default).
+ * No reference counterpart (one kernel form there, src/Mille-feuille-cg.h:97-166). */
+#define MF_SPMV_AUTO (-1)
+#define MF_SPMV_STREAM 0
+#define MF_SPMV_ROWGROUP 1
+mf_status mf_tile_set_spmv_kernel(mf_tiled *t, int kind);
+
 /* Schedule knob for the autotuner: entries per wave of the stand-alone SpMV
  * (its nnz-balanced tile-row ranges are rebuilt; 0 keeps the current value).
  * No reference counterpart (the reference hardcodes its chunking,

@@ -14,8 +14,11 @@
 // Differences: the solve stops at the FP64 relative residual 1e-10 instead of
 // running a fixed 100 (CG) / 50 (BiCGSTAB) iterations; MF_COMPAT_FIXED_ITERS=k
 // in the environment restores a fixed count.  MF_COMPAT_POLICY selects the
-// mixed policy (fp64, rule, errbound, fixed); the *_mixed entry points default
-// to the reference's rule policy.
+// mixed policy (fp64, rule, errbound, fixed, learned); the *_mixed entry
+// points default to the reference's rule policy.  The learned policy loads
+// its MLP from MF_POLICY_FILE, else from policy_mlp.f32 next to this library.
+#include <dlfcn.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -40,7 +43,37 @@ int env_policy(int dflt) {
   if (s == "rule") return MF_POLICY_RULE;
   if (s == "fixed") return MF_POLICY_FIXED;
   if (s == "errbound") return MF_POLICY_ERRBOUND;
+  if (s == "learned") return MF_POLICY_LEARNED;
   return dflt;
+}
+
+// Installs the shipped MLP (1476 little-endian float32) on the handle;
+// without it the learned policy runs the rule policy.
+void install_shipped_policy(mf_tiled *t) {
+  std::string path;
+  if (const char *e = std::getenv("MF_POLICY_FILE")) path = e;
+  if (path.empty()) {
+    Dl_info info;
+    if (dladdr((void *)&install_shipped_policy, &info) && info.dli_fname) {
+      path = info.dli_fname;
+      const size_t slash = path.rfind('/');
+      path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/policy_mlp.f32";
+    }
+  }
+  std::vector<float> p(MF_MLP_NPARAM);
+  FILE *f = path.empty() ? nullptr : std::fopen(path.c_str(), "rb");
+  const bool ok = f && std::fread(p.data(), sizeof(float), p.size(), f) == p.size();
+  if (f) std::fclose(f);
+  if (!ok) {
+    std::printf("mf: no learned policy at %s; using the rule policy\n", path.c_str());
+    return;
+  }
+  float *d = nullptr;
+  if (hipMalloc(&d, sizeof(float) * p.size()) != hipSuccess) return;
+  if (hipMemcpy(d, p.data(), sizeof(float) * p.size(), hipMemcpyHostToDevice) == hipSuccess &&
+      mf_policy_set_mlp(t, d) != MF_OK)
+    std::printf("mf: %s\n", mf_last_error());
+  (void)hipFree(d);
 }
 
 // FP64 check over the tile arrays, as the reference's post-solve loop.
@@ -98,6 +131,7 @@ void compat_cg(int *RowPtr, int *ColIdx, double *Val, float *Val_Low, double *x,
     std::printf("mf: %s\n", mf_last_error());
     return;
   }
+  if (policy == MF_POLICY_LEARNED) install_shipped_policy(t);
   int npad = 0;
   mf_tile_info(t, nullptr, nullptr, nullptr, nullptr, nullptr, &npad);
   DevVec db, dx;
@@ -145,6 +179,7 @@ int compat_bicg(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, i
     std::printf("mf: %s\n", mf_last_error());
     return -1;
   }
+  if (policy == MF_POLICY_LEARNED) install_shipped_policy(t);
   int npad = 0;
   mf_tile_info(t, nullptr, nullptr, nullptr, nullptr, nullptr, &npad);
   DevVec db, dx;

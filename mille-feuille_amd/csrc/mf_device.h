@@ -34,6 +34,7 @@ struct TileView {
   const int *tile_off;  // tilenum+1
   const int *tile_row;  // tilenum
   const uint8_t *rc;    // nnz: (row << 4) | col inside the tile
+  const uint8_t *rp16;  // 16 per tile: row starts inside the tile
   const double *v64;
   const float *v32;
   const uint16_t *v16;
@@ -48,9 +49,9 @@ struct TileView {
 
 inline TileView view_of(const mf_tiled *t) {
   const DeviceLayout &d = t->dev;
-  return TileView{d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc,     d.v64,
-                  d.v32,      d.v16,      d.v8,       d.s16,      d.s8,     d.prec,
-                  t->tilem,   t->tilenum, d.npad};
+  return TileView{d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc,   d.rp16,
+                  d.v64,      d.v32,      d.v16,      d.v8,       d.s16,  d.s8,
+                  d.prec,     t->tilem,   t->tilenum, d.npad};
 }
 
 __device__ __forceinline__ float half_bits_to_float(uint16_t h) {
@@ -360,6 +361,102 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, int tr0, int tr1,
     __builtin_amdgcn_wave_barrier();
     ra = R;
     ta = tb;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row-group tiled SpMV.
+//
+// Lane (g, r) = (lane / 16, lane % 16) computes row r of one tile-row R (the
+// caller's choice; normally four consecutive tile-rows per wave, so lane L
+// holds row 64k + L of a 64-row block).  The lane walks the tiles of its
+// tile-row in batches of kRgBatch: the batch's per-tile data (payload offset,
+// column, the row's start and end bytes, level) is loaded first, then the
+// first kRgUnroll entries of the row in every tile of the batch (payload and
+// column nibble, then the x gathers), so up to kRgBatch*kRgUnroll independent
+// load chains are in flight per lane; longer rows finish in a loop.  The row
+// sum stays in a register (no LDS, no atomics) and is added in a fixed order.
+// Lanes of tile-rows with fewer tiles idle while the wave finishes the
+// longest tile-row of the four: the kernel for matrices whose tile-rows hold
+// similar tile counts (stencils, meshes); irregular ones use wave_spmv.
+// ---------------------------------------------------------------------------
+constexpr int kRgBatch = 4;
+constexpr int kRgUnroll = 2;
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m));
+  return v;
+}
+
+template <int MODE, bool COUNT>
+__device__ __forceinline__ double rowgroup_row(const TileView &A, int R, bool valid, int r,
+                                               const double *__restrict__ x,
+                                               const float *__restrict__ x32,
+                                               const uint8_t *__restrict__ seglev, int force64,
+                                               PrecCount *cnt) {
+  int t0 = 0, nt = 0;
+  if (valid) {
+    t0 = A.tile_ptr[R];
+    nt = A.tile_ptr[R + 1] - t0;
+  }
+  const int ntw = wave_max(nt);
+  double acc = 0.0;
+  for (int i = 0; i < ntw; i += kRgBatch) {
+    int off[kRgBatch], xo[kRgBatch], b[kRgBatch], e[kRgBatch], lv[kRgBatch];
+    float sc[kRgBatch];
+#pragma unroll
+    for (int u = 0; u < kRgBatch; ++u) {
+      off[u] = xo[u] = b[u] = e[u] = 0;
+      lv[u] = MF_PREC_FP64;
+      sc[u] = 1.0f;
+      if (i + u < nt) {
+        const int t = t0 + i + u;
+        off[u] = A.tile_off[t];
+        const int nz = A.tile_off[t + 1] - off[u];
+        xo[u] = A.tile_col[t] << 4;
+        b[u] = A.rp16[(size_t)t * 16 + r];
+        e[u] = r < 15 ? A.rp16[(size_t)t * 16 + r + 1] : nz;
+        if (MODE != kAll64) {
+          lv[u] = tile_level<MODE>(A, seglev, t, force64);
+          sc[u] = lv[u] == MF_PREC_FP16 ? A.s16[t] : A.s8[t];
+          if (COUNT && r == 0) cnt->add(lv[u]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kRgUnroll; ++q)
+#pragma unroll
+      for (int u = 0; u < kRgBatch; ++u)
+        if (b[u] + q < e[u]) {
+          const int k = off[u] + b[u] + q;
+          acc += entry_product<MODE>(A, lv[u], k, xo[u] + (A.rc[k] & 15), x, x32, sc[u]);
+        }
+#pragma unroll
+    for (int u = 0; u < kRgBatch; ++u)
+      for (int k = off[u] + b[u] + kRgUnroll; k < off[u] + e[u]; ++k)
+        acc += entry_product<MODE>(A, lv[u], k, xo[u] + (A.rc[k] & 15), x, x32, sc[u]);
+  }
+  return acc;
+}
+
+// K passes of four tile-rows: w[k] = row 16*tr0 + 64k + lane of A x (0 past
+// tr1).  The wave's state layout (lane L holds rows L + 64k) is the pass
+// layout, so the products land in the registers that need them.
+template <int MODE, bool COUNT, int K>
+__device__ __forceinline__ void rowgroup_passes(const TileView &A, int tr0, int tr1, int lane,
+                                                const double *__restrict__ x,
+                                                const float *__restrict__ x32,
+                                                const uint8_t *__restrict__ seglev, int force64,
+                                                PrecCount *cnt, double (&w)[K]) {
+  const int g = lane >> 4, r = lane & 15;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    w[k] = 0.0;
+    if (tr0 + 4 * k < tr1) {  // wave-uniform
+      const int R = tr0 + 4 * k + g;
+      w[k] = rowgroup_row<MODE, COUNT>(A, R, R < tr1, r, x, x32, seglev, force64, cnt);
+    }
   }
 }
 

@@ -27,6 +27,9 @@ struct DeviceLayout {
   int *tile_off = nullptr;     // tilenum+1 (payload offset; == csr_offset)
   int *tile_row = nullptr;     // tilenum (tile-row of each tile)
   uint8_t *rc = nullptr;       // csrsize (+pad): (row << 4) | col inside the tile
+  uint8_t *rp16 = nullptr;     // 16*tilenum: row starts inside each tile
+  int max_tiles_per_row = 0;   // largest tile count of a tile-row
+  int spmv_kernel = 0;         // MF_SPMV_STREAM or MF_SPMV_ROWGROUP
   double *v64 = nullptr;       // csrsize FP64 payload
   float *v32 = nullptr;        // csrsize FP32 payload
   uint16_t *v16 = nullptr;     // csrsize FP16 payload (bits), scaled by s16
@@ -83,4 +86,7 @@ mf_status upload_tiles(mf_tiled *t, int device);
 void free_device(mf_tiled *t);
 // nnz-balanced contiguous tile-row partition for `nwaves` waves.
 std::vector<int> balance_tilerows(const mf_tiled *t, int nwaves);
+// Row-group kernel when every tile-row holds at most a few times the mean
+// tile count (regular sparsity), else the streamed kernel.
+int default_spmv_kernel(const mf_tiled *t);
 }  // namespace mf

@@ -536,20 +536,22 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
 // needs two grid syncs instead of three:
 //   beta = gamma/gamma_old, alpha = gamma/(delta - beta*gamma/alpha_old),
 //   p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s   (s = A p).
-// Every wave owns 16*(tr1-tr0) <= 64*K consecutive rows and keeps p and s of
-// those rows in registers (lane L holds rows L, L+64, ...) and x in
-// wave-private LDS; the SpMV epilogue drops w into wave-private LDS.  The only
-// vector stored per iteration is r (plus its FP32 copy for low-precision
-// tiles), which the next SpMV gathers; the wave reads its own rows of r back
-// (L2-resident: the SpMV has just gathered them) instead of holding a third
-// register vector.  FP64 residual recomputation replaces r by b - A x and
+// Every wave owns 16*(tr1-tr0) <= 64*K consecutive rows and keeps p, s and
+// w = A r of those rows in registers (lane L holds rows L, L+64, ...) and x
+// in wave-private LDS.  The SpMV is the row-group form, whose pass k leaves
+// row 64k + L of the wave's block in lane L: the products land directly in
+// w[k].  The only vector stored per iteration is r (plus its FP32 copy for
+// low-precision tiles), which the next SpMV gathers; the wave reads its own
+// rows of r back (L2-resident: the SpMV has just gathered them).  FP64 residual recomputation replaces r by b - A x and
 // s by A p, both in FP64, which keeps the recurrences consistent without
 // restarting (residual replacement); a stagnating mixed run falls back to
 // FP64 tiles and restarts from the recomputed residual.
 // ---------------------------------------------------------------------------
-constexpr int kResBlock = 512;  // one block per CU, 8 waves, up to 256 VGPRs
-constexpr int kResWaves = kResBlock / 64;
-constexpr int kResK = 16;        // rows per lane: up to 1024 rows per wave
+// Two shapes, one block per CU: 16 waves with 8 rows per lane (up to 128
+// VGPRs; 8192 rows per CU) or 8 waves with 16 rows per lane (up to 256
+// VGPRs; 8192 rows per CU as well, for kernels that need the registers).
+constexpr int kResBlockA = 1024, kResKA = 8;
+constexpr int kResBlockB = 512, kResKB = 16;
 
 // A copy of v the compiler cannot see through: per-row addresses derived from
 // it are recomputed where they are used instead of being hoisted out of the
@@ -560,34 +562,29 @@ __device__ __forceinline__ int opaque(int v) {
   return v;
 }
 
-template <int MODE, int K>
-__global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
-  __shared__ double lds[kResWaves];
+template <int MODE, int K, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_cg_res(SolveArgs a) {
+  constexpr int NW = BLOCK / 64;
+  __shared__ double lds[NW];
   __shared__ int lds_i;
-  __shared__ WaveLds wl[kResWaves];
-  __shared__ double wbuf[kResWaves * K * 64];
-  __shared__ double xbuf[kResWaves * K * 64];
+  __shared__ double xbuf[NW * K * 64];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int wid = blockIdx.x * kResWaves + wv;
+  const int wid = blockIdx.x * NW + wv;
   const int tr0 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid]);
   const int tr1 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid + 1]);
   const int row0 = tr0 * 16;
   const int nrows = (tr1 - tr0) * 16;
-  double *W = wbuf + wv * K * 64;
   double *X = xbuf + wv * K * 64;  // x of the wave's rows
-  WaveLds &L = wl[wv];
   constexpr bool mixed = MODE != kAll64;
   const bool learned = MODE == kPerTile && a.mlp != nullptr;
   unsigned epoch = 0;
   int slot = 0;
   PrecCount cnt;
-  double p[K], s[K];
-  auto epi_w = [&](const int i, const double v) { W[i - row0] = v; };
+  double p[K], s[K], w[K];
 
   // r = b - A x0
-  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, L, lane, nullptr, epi_w);
-  __builtin_amdgcn_wave_barrier();
+  rowgroup_passes<kAll64, false, K>(a.A, tr0, tr1, lane, a.x, nullptr, nullptr, 0, nullptr, w);
   double red[2] = {0.0, 0.0};
   const int ln0 = opaque(lane);
 #pragma unroll
@@ -599,7 +596,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
       const int i = row0 + li;
       const double bv = a.b[i];
       X[li] = a.x[i];
-      const double rv = bv - W[li];
+      const double rv = bv - w[k];
       red[0] += rv * rv;
       red[1] += bv * bv;
       a.r[i] = rv;
@@ -630,8 +627,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
       // from the segment norms of r written before that sync.
       if (infer && !force64) wave_infer(a.A, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
       infer = false;
-      wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.r, a.p32, a.seg_p, force64, L, lane, &cnt, epi_w);
-      __builtin_amdgcn_wave_barrier();
+      rowgroup_passes<MODE, mixed, K>(a.A, tr0, tr1, lane, a.r, a.p32, a.seg_p, force64, &cnt, w);
       double gd[2] = {0.0, 0.0};
       const int ln1 = opaque(lane);
 #pragma unroll
@@ -640,7 +636,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
         if (li < nrows) {
           const double rv = a.r[row0 + li];
           gd[0] += rv * rv;
-          gd[1] += W[li] * rv;
+          gd[1] += w[k] * rv;
         }
       }
       grid_sum<2>(gd, a.partials, a.gs, epoch, slot, lds, &lds_i);
@@ -665,8 +661,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
         if (li < nrows) a.x[row0 + li] = X[li];
       }
       grid_sync(a.gs, epoch, &lds_i);
-      wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, L, lane, nullptr, epi_w);
-      __builtin_amdgcn_wave_barrier();
+      rowgroup_passes<kAll64, false, K>(a.A, tr0, tr1, lane, a.x, nullptr, nullptr, 0, nullptr, w);
       double tr[1] = {0.0};
       const int ln3 = opaque(lane);
 #pragma unroll
@@ -674,7 +669,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
         const int li = ln3 + 64 * k;
         if (li < nrows) {
           const int i = row0 + li;
-          const double rv = a.b[i] - W[li];
+          const double rv = a.b[i] - w[k];
           tr[0] += rv * rv;
           a.r[i] = rv;
           if (mixed) a.p32[i] = (float)rv;
@@ -704,14 +699,9 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
           if (li < nrows) a.q[row0 + li] = p[k];
         }
         grid_sync(a.gs, epoch, &lds_i);
-        wave_spmv<kAll64, false>(a.A, tr0, tr1, a.q, nullptr, nullptr, 0, L, lane, nullptr, epi_w);
-        __builtin_amdgcn_wave_barrier();
-        const int ln5 = opaque(lane);
+        rowgroup_passes<kAll64, false, K>(a.A, tr0, tr1, lane, a.q, nullptr, nullptr, 0, nullptr, w);
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const int li = ln5 + 64 * k;
-          if (li < nrows) s[k] = W[li];
-        }
+        for (int k = 0; k < K; ++k) s[k] = w[k];
       }
       continue;
     }
@@ -750,7 +740,7 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
         const int i = row0 + li;
         const double rv = a.r[i];
         p[k] = rv + beta * p[k];
-        s[k] = W[li] + beta * s[k];
+        s[k] = w[k] + beta * s[k];
         X[li] += alpha * p[k];
         const double rn = rv - alpha * s[k];
         d = fabs(rn - rv);
@@ -781,11 +771,16 @@ __global__ __launch_bounds__(kResBlock) void k_cg_res(SolveArgs a) {
   }
   grid_sync(a.gs, epoch, &lds_i);
   double fin[1] = {0.0};
-  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, L, lane, nullptr,
-                           [&](const int i, const double v) {
-                             const double d = a.b[i] - v;
-                             fin[0] += d * d;
-                           });
+  rowgroup_passes<kAll64, false, K>(a.A, tr0, tr1, lane, a.x, nullptr, nullptr, 0, nullptr, w);
+  const int ln8 = opaque(lane);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int li = ln8 + 64 * k;
+    if (li < nrows) {
+      const double d = a.b[row0 + li] - w[k];
+      fin[0] += d * d;
+    }
+  }
   grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.stats[kStIters] = it;
@@ -939,20 +934,37 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   int G = 0, block = kBlock;
   mf_status sst = MF_OK;
   bool resident = false;
-  if (!bicg && std::getenv("MF_NO_RESIDENT") == nullptr) {
-    // Register-resident CG when every wave's even share of rows fits 64*K.
-    kern = mode == kAll64 ? (const void *)k_cg_res<kAll64, kResK>
-         : mode == kPerSeg ? (const void *)k_cg_res<kPerSeg, kResK>
-                           : (const void *)k_cg_res<kPerTile, kResK>;
-    sst = grid_for(kern, opt.blocks, &G, kResBlock, true);
-    if (sst != MF_OK) return sst;
-    const int nw = G * kResWaves;
-    const int q = (t->tilem + nw - 1) / nw;
-    if (16 * q <= 64 * kResK) {
-      resident = true;
-      block = kResBlock;
-      sst = ensure_schedule(t, nw, true);
+  if (!bicg && t->dev.spmv_kernel == MF_SPMV_ROWGROUP && std::getenv("MF_NO_RESIDENT") == nullptr) {
+    // Resident CG when every wave's even share of rows fits its registers:
+    // 16 waves x 512 rows per CU unless that kernel spills, else 8 waves x
+    // 1024 rows.
+    const void *ka = mode == kAll64 ? (const void *)k_cg_res<kAll64, kResKA, kResBlockA>
+                   : mode == kPerSeg ? (const void *)k_cg_res<kPerSeg, kResKA, kResBlockA>
+                                     : (const void *)k_cg_res<kPerTile, kResKA, kResBlockA>;
+    const void *kb = mode == kAll64 ? (const void *)k_cg_res<kAll64, kResKB, kResBlockB>
+                   : mode == kPerSeg ? (const void *)k_cg_res<kPerSeg, kResKB, kResBlockB>
+                                     : (const void *)k_cg_res<kPerTile, kResKB, kResBlockB>;
+    const struct { const void *k; int block, K; } shapes[2] = {{ka, kResBlockA, kResKA},
+                                                                {kb, kResBlockB, kResKB}};
+    for (const auto &sh : shapes) {
+      // a shape whose kernel spills registers to scratch is passed over
+      hipFuncAttributes fa;
+      MF_HIP(hipFuncGetAttributes(&fa, sh.k));
+      if (fa.localSizeBytes > 0 && &sh != &shapes[1]) continue;
+      int Gs = 0;
+      sst = grid_for(sh.k, opt.blocks, &Gs, sh.block, true);
       if (sst != MF_OK) return sst;
+      const int nw = Gs * (sh.block / 64);
+      const int q = (t->tilem + nw - 1) / nw;
+      if (q <= 4 * sh.K) {  // 16 q rows <= 64 K
+        resident = true;
+        kern = sh.k;
+        G = Gs;
+        block = sh.block;
+        sst = ensure_schedule(t, nw, true);
+        if (sst != MF_OK) return sst;
+        break;
+      }
     }
   }
   if (!resident) {

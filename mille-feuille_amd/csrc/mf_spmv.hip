@@ -1,8 +1,10 @@
-// mf_spmv.hip -- stand-alone tiled SpMV y = A x (streamed, wave64, no global
-// atomics).
+// mf_spmv.hip -- stand-alone tiled SpMV y = A x (wave64, no global atomics),
+// in two forms: streamed (wave_spmv: nnz-balanced tile-row ranges, LDS row
+// accumulators; any matrix) and row-group (rowgroup_row: one row per lane,
+// register sums; tile-rows of similar tile counts).
 //
-// Launch shape: 256-thread blocks, one nnz-balanced range of tile-rows per
-// wave (about 1024 entries).  Blocks are dispatched round-robin over the 8
+// Launch shape: 256-thread blocks; streamed: one nnz-balanced range of
+// tile-rows per wave (about 512 entries); row-group: four tile-rows per wave.  Blocks are dispatched round-robin over the 8
 // XCDs (block b runs on XCD b % 8), so the block index is remapped such that
 // each XCD walks one contiguous eighth of the matrix: neighbouring tile-rows
 // share x segments, and keeping them on one XCD keeps those segments in that
@@ -26,6 +28,22 @@ __global__ __launch_bounds__(256) void k_spmv(TileView A, const int *__restrict_
                          lane, nullptr, [&](const int i, const double v) { y[i] = v; });
 }
 
+// Row-group kernel: four consecutive tile-rows per wave, one row per lane,
+// row sums in registers (rowgroup_row).  Same XCD-contiguous block order.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_spmv_rg(TileView A, const double *__restrict__ x,
+                                                 const float *__restrict__ x32,
+                                                 double *__restrict__ y) {
+  const int per_xcd = gridDim.x >> 3;
+  const int lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  const int lane = threadIdx.x & 63;
+  const int R0 = (lb * 4 + (threadIdx.x >> 6)) * 4;
+  if (R0 >= A.tilem) return;  // wave-uniform
+  const int R = R0 + (lane >> 4);
+  const double v = rowgroup_row<MODE, false>(A, R, R < A.tilem, lane & 15, x, x32, nullptr, 0, nullptr);
+  if (R < A.tilem) y[16 * R + (lane & 15)] = v;
+}
+
 // Zero the padding of y beyond the tile-rows (npad may exceed 16*tilem when
 // the matrix has more tile columns than tile-rows).
 __global__ void k_zero_tail(double *y, int from, int to) {
@@ -36,14 +54,26 @@ __global__ void k_zero_tail(double *y, int from, int to) {
 mf_status launch_spmv(const mf_tiled *t, const double *d_x, const float *d_x32, double *d_y,
                       int mode, hipStream_t s) {
   const TileView A = view_of(t);
-  const int nw = t->dev.spmv_nwaves;
-  const int blocks = ((nw + 3) / 4 + 7) / 8 * 8;
-  if (A.tilem > 0 && nw > 0) {
-    if (mode == kAll64)
-      k_spmv<kAll64><<<blocks, 256, 0, s>>>(A, t->dev.spmv_tr, nw, d_x, d_x32, d_y);
-    else
-      k_spmv<kPerTile><<<blocks, 256, 0, s>>>(A, t->dev.spmv_tr, nw, d_x, d_x32, d_y);
-    MF_HIP(hipGetLastError());
+  if (t->dev.spmv_kernel == MF_SPMV_ROWGROUP) {
+    const int nwv = (A.tilem + 3) / 4;
+    const int blocks = ((nwv + 3) / 4 + 7) / 8 * 8;
+    if (A.tilem > 0) {
+      if (mode == kAll64)
+        k_spmv_rg<kAll64><<<blocks, 256, 0, s>>>(A, d_x, d_x32, d_y);
+      else
+        k_spmv_rg<kPerTile><<<blocks, 256, 0, s>>>(A, d_x, d_x32, d_y);
+      MF_HIP(hipGetLastError());
+    }
+  } else {
+    const int nw = t->dev.spmv_nwaves;
+    const int blocks = ((nw + 3) / 4 + 7) / 8 * 8;
+    if (A.tilem > 0 && nw > 0) {
+      if (mode == kAll64)
+        k_spmv<kAll64><<<blocks, 256, 0, s>>>(A, t->dev.spmv_tr, nw, d_x, d_x32, d_y);
+      else
+        k_spmv<kPerTile><<<blocks, 256, 0, s>>>(A, t->dev.spmv_tr, nw, d_x, d_x32, d_y);
+      MF_HIP(hipGetLastError());
+    }
   }
   const int from = 16 * A.tilem, to = A.npad;
   if (to > from) {
@@ -82,6 +112,15 @@ extern "C" mf_status mf_stream_copy(const double *d_src, double *d_dst, int64_t 
     k_copy_tail<<<1, 64, 0, s>>>(d_src, d_dst, 4 * n4, n);
     MF_HIP(hipGetLastError());
   }
+  return MF_OK;
+}
+
+extern "C" mf_status mf_tile_set_spmv_kernel(mf_tiled *t, int kind) {
+  if (!t) return fail(MF_ERR_ARG, "mf_tile_set_spmv_kernel: null handle");
+  if (kind == MF_SPMV_AUTO) kind = default_spmv_kernel(t);
+  if (kind != MF_SPMV_STREAM && kind != MF_SPMV_ROWGROUP)
+    return fail(MF_ERR_ARG, "mf_tile_set_spmv_kernel: unknown kernel");
+  t->dev.spmv_kernel = kind;
   return MF_OK;
 }
 

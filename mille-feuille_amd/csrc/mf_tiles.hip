@@ -104,16 +104,22 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   if (d.npad == 0) d.npad = kTile;
 
   // Per entry: (row << 4) | col inside the tile, from the tile's row starts.
+  // Per tile: the 16 row starts (rows past the last row of a short final
+  // tile-row start at the tile's nnz, which is then <= 15*16).
   std::vector<int> tile_row(T);
   std::vector<uint8_t> rc((size_t)Z + kPad, 0);
+  std::vector<uint8_t> rp16((size_t)T * kTile + kPad, 0);
   std::vector<int> diag(TM, -1);
+  int max_tiles = 0;
   for (int bi = 0; bi < TM; ++bi) {
     const int h = bi == TM - 1 ? t->rowA - (TM - 1) * kTile : kTile;
+    max_tiles = std::max(max_tiles, t->tile_ptr[bi + 1] - t->tile_ptr[bi]);
     for (int tt = t->tile_ptr[bi]; tt < t->tile_ptr[bi + 1]; ++tt) {
       tile_row[tt] = bi;
       if (t->tile_columnidx[tt] == bi) diag[bi] = tt;
       const int off = t->csr_offset[tt], nzt = t->csr_offset[tt + 1] - off;
       const unsigned char *rp = &t->Blockcsr_Ptr[t->csrptr_offset[tt]];
+      for (int r = 0; r < kTile; ++r) rp16[(size_t)tt * kTile + r] = (uint8_t)(r < h ? rp[r] : nzt);
       for (int r = 0; r < h; ++r) {
         const int e = r == h - 1 ? nzt : rp[r + 1];
         for (int k = rp[r]; k < e; ++k)
@@ -121,6 +127,8 @@ mf_status upload_tiles(mf_tiled *t, int device) {
       }
     }
   }
+  d.max_tiles_per_row = max_tiles;
+  d.spmv_kernel = default_spmv_kernel(t);
   // nnz-balanced tile-row ranges for the stand-alone SpMV, about
   // kSpmvWaveNnz entries per wave (retuned by mf_tile_set_schedule).
   const int nw = std::max(1, (int)((Z + kSpmvWaveNnz - 1) / kSpmvWaveNnz));
@@ -131,6 +139,7 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_TRY(dalloc(&d.tile_off, T + 1));
   MF_TRY(dalloc(&d.tile_row, T));
   MF_TRY(dalloc(&d.rc, (size_t)Z + kPad));
+  MF_TRY(dalloc(&d.rp16, (size_t)T * kTile + kPad));
   MF_TRY(dalloc(&d.v64, (size_t)Z + kPad));
   MF_TRY(dalloc(&d.v32, (size_t)Z + kPad));
   MF_TRY(dalloc(&d.v16, (size_t)Z + kPad));
@@ -149,6 +158,7 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_HIP(hipMemcpy(d.tile_off, t->csr_offset.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.tile_row, tile_row.data(), sizeof(int) * T, hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.rc, rc.data(), (size_t)Z + kPad, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.rp16, rp16.data(), rp16.size(), hipMemcpyHostToDevice));
   MF_HIP(hipMemset(d.v64, 0, sizeof(double) * ((size_t)Z + kPad)));
   MF_HIP(hipMemset(d.v32, 0, sizeof(float) * ((size_t)Z + kPad)));
   MF_HIP(hipMemset(d.v16, 0, sizeof(uint16_t) * ((size_t)Z + kPad)));
@@ -170,10 +180,17 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   return MF_OK;
 }
 
+int default_spmv_kernel(const mf_tiled *t) {
+  const int TM = t->tilem;
+  if (TM == 0) return MF_SPMV_STREAM;
+  const double mean = (double)t->tilenum / TM;
+  return t->dev.max_tiles_per_row <= 4.0 * mean + 8.0 ? MF_SPMV_ROWGROUP : MF_SPMV_STREAM;
+}
+
 void free_device(mf_tiled *t) {
   DeviceLayout &d = t->dev;
   if (d.device >= 0) (void)hipSetDevice(d.device);
-  void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.spmv_tr,
+  void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.rp16, d.spmv_tr,
                   d.v64,      d.v32,      d.v16,      d.v8,       d.s16,    d.s8,
                   d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp, d.feat16,
                   t->sched.wave_tr, t->work};

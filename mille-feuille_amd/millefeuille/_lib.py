@@ -23,6 +23,7 @@ MF_OK = 0
 MF_NOT_CONVERGED = 1
 PREC_FP64, PREC_FP32, PREC_FP16, PREC_INT8, PREC_SKIP = range(5)
 POLICY_FP64, POLICY_RULE, POLICY_FIXED, POLICY_LEARNED, POLICY_ERRBOUND = range(5)
+SPMV_AUTO, SPMV_STREAM, SPMV_ROWGROUP = -1, 0, 1
 MLP_NPARAM = 32 * 8 + 32 + 32 * 32 + 32 + 4 * 32 + 4
 
 
@@ -102,6 +103,7 @@ SIGNATURES = {
     "mf_spmv": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "mf_stream_copy": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
     "mf_tile_set_schedule": (C.c_int, [_vp, C.c_int]),
+    "mf_tile_set_spmv_kernel": (C.c_int, [_vp, C.c_int]),
     "mf_solve_opts_default": (None, [C.POINTER(MfSolveOpts)]),
     "mf_cg_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(MfSolveOpts), C.POINTER(MfReport)]),
     "mf_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(MfSolveOpts), C.POINTER(MfReport)]),

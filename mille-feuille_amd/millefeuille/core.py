@@ -145,6 +145,11 @@ class TiledMatrix:
     def set_schedule(self, spmv_wave_nnz):
         check("mf_tile_set_schedule", lib.mf_tile_set_schedule(self.h, int(spmv_wave_nnz)))
 
+    def set_spmv_kernel(self, kind):
+        """'auto', 'stream' or 'rowgroup' (or the MF_SPMV_* value)."""
+        k = {"auto": _lib.SPMV_AUTO, "stream": _lib.SPMV_STREAM, "rowgroup": _lib.SPMV_ROWGROUP}
+        check("mf_tile_set_spmv_kernel", lib.mf_tile_set_spmv_kernel(self.h, k.get(kind, kind)))
+
     def get_precision(self):
         out = np.zeros(max(self.tilenum, 1), np.uint8)
         check("mf_tile_get_precision", lib.mf_tile_get_precision(self.h, _p(out)))
