@@ -118,12 +118,28 @@ def main():
     gbs = bytes_spmv / (t_spmv * 1e-3) / 1e9
     log(f"spmv fp64: {t_spmv * 1e3:.1f} us median, {bytes_spmv / 1e6:.1f} MB -> {gbs:.0f} GB/s "
         f"({100 * gbs / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+    # --- copy calibration: a plain device copy of known size (its counter
+    # readings scale FETCH_SIZE / WRITE_SIZE of the SpMV into HBM bytes) -----
+    from millefeuille._lib import lib
+    ncopy = 1 << 25  # 32 Mi doubles: 256 MiB read + 256 MiB written
+    csrc, cdst = mf.DeviceArray(ncopy, np.float64), mf.DeviceArray(ncopy, np.float64)
+    csrc.zero()
+    for _ in range(3):
+        lib.mf_stream_copy(csrc.ptr, cdst.ptr, ncopy, None)
+    ctimes = []
+    for _ in range(10):
+        timer.start()
+        lib.mf_stream_copy(csrc.ptr, cdst.ptr, ncopy, None)
+        ctimes.append(timer.stop())
+    copy_gbs = 2 * 8 * ncopy / (float(np.median(ctimes)) * 1e-3) / 1e9
+    del csrc, cdst
+    log(f"copy: {copy_gbs:.0f} GB/s (256 MiB read + 256 MiB written)")
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "spmv_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("grid") == args.grid:
+        if tr.get("grid") == args.grid and tr.get("kernel") == tm.spmv_kernel_name():
             traffic = tr.get("hbm_bytes_per_launch")
 
     # --- CPU baseline: the oracle's FP64 CG (1 thread) to the same tolerance,
@@ -170,7 +186,8 @@ def main():
         "fp64": {"ms": round(fp64_ms, 3), "kernel_ms": round(rep64["time_ms"], 3),
                  "iterations": rep64["iterations"], "rel_residual_fp64_cpu": rel64},
         "speedup_vs_fp64": round(fp64_ms / mixed_ms, 3),
-        "spmv": {"us_median": round(t_spmv * 1e3, 2), "bytes": bytes_spmv, "GBps": round(gbs, 1)},
+        "spmv": {"us_median": round(t_spmv * 1e3, 2), "bytes": bytes_spmv, "GBps": round(gbs, 1),
+                 "kernel": tm.spmv_kernel_name(), "copy_GBps_measured": round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": {"value": round(cpu_ms, 1), "unit": "ms", "cores": 1, "kind": "port",

@@ -168,9 +168,9 @@ mf_status mf_spmv(mf_tiled *t, const double *d_x, const float *d_x32,
                   double *d_y, int mode, void *stream);
 
 /* Stand-alone SpMV kernel of a handle: MF_SPMV_STREAM (nnz-balanced tile
- * ranges, LDS row accumulators; any sparsity), MF_SPMV_ROWGROUP (one row per
- * lane, four tile-rows per wave; regular sparsity) or MF_SPMV_AUTO (row-group
- * when no tile-row holds more than 4x the mean tile count + 8; the default).
+ * ranges, LDS row accumulators; any sparsity; the default), MF_SPMV_ROWGROUP
+ * (one row per lane, four tile-rows per wave; regular sparsity) or
+ * MF_SPMV_AUTO (the default).
  * No reference counterpart (one kernel form there, src/Mille-feuille-cg.h:97-166). */
 #define MF_SPMV_AUTO (-1)
 #define MF_SPMV_STREAM 0
@@ -215,8 +215,16 @@ typedef struct mf_solve_opts {
   int blocks;           /* persistent grid size override (0 = occupancy) */
   int fixed_iters;      /* >0: run exactly this many iterations, no test
                            (reference timing mode, src/Mille-feuille-cg.h:426) */
+  int form;             /* CG kernel form: MF_FORM_AUTO, MF_FORM_MEMORY (vectors
+                           in HBM, three syncs per iteration) or MF_FORM_RESIDENT
+                           (vectors in registers/LDS, two syncs; needs regular
+                           tile-rows and at most 8192 rows per CU) */
   void *stream;         /* hipStream_t */
 } mf_solve_opts;
+
+#define MF_FORM_AUTO 0
+#define MF_FORM_MEMORY 1
+#define MF_FORM_RESIDENT 2
 
 typedef struct mf_report {
   int iterations;       /* CG / BiCGSTAB iterations executed */
@@ -228,6 +236,7 @@ typedef struct mf_report {
   double time_ms;       /* kernel time (HIP events) */
   long long prec_hist[5]; /* tile-iterations per precision level */
   int grid_blocks;      /* persistent grid actually launched */
+  int form_used;        /* MF_FORM_MEMORY or MF_FORM_RESIDENT */
   int policy_used;      /* MF_POLICY_* that ran (LEARNED without an installed
                            MLP runs the rule policy) */
 } mf_report;

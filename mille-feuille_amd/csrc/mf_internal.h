@@ -86,7 +86,9 @@ mf_status upload_tiles(mf_tiled *t, int device);
 void free_device(mf_tiled *t);
 // nnz-balanced contiguous tile-row partition for `nwaves` waves.
 std::vector<int> balance_tilerows(const mf_tiled *t, int nwaves);
-// Row-group kernel when every tile-row holds at most a few times the mean
-// tile count (regular sparsity), else the streamed kernel.
+// Default stand-alone SpMV kernel (MF_SPMV_STREAM or MF_SPMV_ROWGROUP).
 int default_spmv_kernel(const mf_tiled *t);
+// No tile-row holds more than 4x the mean tile count + 8 (the row-group
+// kernel, one tile-row per 16 lanes, wastes little on such matrices).
+bool regular_tilerows(const mf_tiled *t);
 }  // namespace mf

@@ -934,7 +934,12 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   int G = 0, block = kBlock;
   mf_status sst = MF_OK;
   bool resident = false;
-  if (!bicg && t->dev.spmv_kernel == MF_SPMV_ROWGROUP && std::getenv("MF_NO_RESIDENT") == nullptr) {
+  if (opt.form < MF_FORM_AUTO || opt.form > MF_FORM_RESIDENT) return fail(MF_ERR_ARG, "solve: unknown form");
+  // Auto picks the memory form: on MI355X (3D 7-point 128^3) it runs FP64 CG
+  // as fast as the resident form (106 vs 108 us per iteration) and mixed CG
+  // faster (152 vs 213): the resident form's 8-16 waves per CU leave the SpMV
+  // latency-bound.
+  if (!bicg && opt.form == MF_FORM_RESIDENT && regular_tilerows(t)) {
     // Resident CG when every wave's even share of rows fits its registers:
     // 16 waves x 512 rows per CU unless that kernel spills, else 8 waves x
     // 1024 rows.
@@ -1075,6 +1080,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   r.time_ms = ms;
   for (int i = 0; i < 5; ++i) r.prec_hist[i] = (long long)hist[i];
   r.grid_blocks = G;
+  r.form_used = resident ? MF_FORM_RESIDENT : MF_FORM_MEMORY;
   r.policy_used = opt.policy;
   int ret = status;
   if (ret == MF_OK && opt.fixed_iters <= 0 && !(r.rel_residual <= opt.tol)) ret = MF_NOT_CONVERGED;

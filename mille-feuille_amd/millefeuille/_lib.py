@@ -24,6 +24,7 @@ MF_NOT_CONVERGED = 1
 PREC_FP64, PREC_FP32, PREC_FP16, PREC_INT8, PREC_SKIP = range(5)
 POLICY_FP64, POLICY_RULE, POLICY_FIXED, POLICY_LEARNED, POLICY_ERRBOUND = range(5)
 SPMV_AUTO, SPMV_STREAM, SPMV_ROWGROUP = -1, 0, 1
+FORM_AUTO, FORM_MEMORY, FORM_RESIDENT = 0, 1, 2
 MLP_NPARAM = 32 * 8 + 32 + 32 * 32 + 32 + 4 * 32 + 4
 
 
@@ -46,7 +47,7 @@ class MfSolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("policy", C.c_int),
                 ("check_interval", C.c_int), ("refresh_ratio", C.c_double),
                 ("max_refresh", C.c_int), ("refresh_every", C.c_int), ("blocks", C.c_int),
-                ("fixed_iters", C.c_int),
+                ("fixed_iters", C.c_int), ("form", C.c_int),
                 ("stream", C.c_void_p)]
 
 
@@ -55,7 +56,7 @@ class MfReport(C.Structure):
                 ("status", C.c_int), ("rel_residual", C.c_double),
                 ("rel_residual_recursive", C.c_double), ("time_ms", C.c_double),
                 ("prec_hist", C.c_longlong * 5), ("grid_blocks", C.c_int),
-                ("policy_used", C.c_int)]
+                ("form_used", C.c_int), ("policy_used", C.c_int)]
 
     def as_dict(self):
         return {"iterations": self.iterations, "refreshes": self.refreshes,
@@ -63,7 +64,8 @@ class MfReport(C.Structure):
                 "rel_residual": self.rel_residual,
                 "rel_residual_recursive": self.rel_residual_recursive,
                 "time_ms": self.time_ms, "prec_hist": list(self.prec_hist),
-                "grid_blocks": self.grid_blocks, "policy_used": self.policy_used}
+                "grid_blocks": self.grid_blocks, "form_used": self.form_used,
+                "policy_used": self.policy_used}
 
 
 _vp = C.c_void_p

@@ -35,8 +35,11 @@ def read_mtx(path):
         lib.mf_csr_free(C.byref(a))
 
 
+_FORMS = {"auto": _lib.FORM_AUTO, "memory": _lib.FORM_MEMORY, "resident": _lib.FORM_RESIDENT}
+
+
 def solve_opts(tol=1e-10, maxiter=10000, policy="fp64", check_interval=8, refresh_ratio=0.1,
-               max_refresh=64, refresh_every=64, blocks=0, fixed_iters=0):
+               max_refresh=64, refresh_every=64, blocks=0, fixed_iters=0, form="auto"):
     o = MfSolveOpts()
     lib.mf_solve_opts_default(C.byref(o))
     o.tol = tol
@@ -48,6 +51,7 @@ def solve_opts(tol=1e-10, maxiter=10000, policy="fp64", check_interval=8, refres
     o.refresh_every = refresh_every
     o.blocks = blocks
     o.fixed_iters = fixed_iters
+    o.form = _FORMS[form] if isinstance(form, str) else int(form)
     return o
 
 
@@ -69,6 +73,7 @@ class TiledMatrix:
         vals = [C.c_int() for _ in range(6)]
         check("mf_tile_info", lib.mf_tile_info(h, *[C.byref(v) for v in vals]))
         self.tilem, self.tilen, self.tilenum, self.csrsize, self.csrptrlen, self.npad = [v.value for v in vals]
+        self._spmv_kernel = _lib.SPMV_STREAM
 
     def free(self):
         if getattr(self, "h", None):
@@ -145,10 +150,16 @@ class TiledMatrix:
     def set_schedule(self, spmv_wave_nnz):
         check("mf_tile_set_schedule", lib.mf_tile_set_schedule(self.h, int(spmv_wave_nnz)))
 
+    def spmv_kernel_name(self):
+        """Device kernel that mf_spmv launches for this handle (mode 0)."""
+        return "mf::k_spmv_rg<0>" if self._spmv_kernel == _lib.SPMV_ROWGROUP else "mf::k_spmv<0>"
+
     def set_spmv_kernel(self, kind):
         """'auto', 'stream' or 'rowgroup' (or the MF_SPMV_* value)."""
         k = {"auto": _lib.SPMV_AUTO, "stream": _lib.SPMV_STREAM, "rowgroup": _lib.SPMV_ROWGROUP}
-        check("mf_tile_set_spmv_kernel", lib.mf_tile_set_spmv_kernel(self.h, k.get(kind, kind)))
+        kind = k.get(kind, kind)
+        check("mf_tile_set_spmv_kernel", lib.mf_tile_set_spmv_kernel(self.h, kind))
+        self._spmv_kernel = _lib.SPMV_STREAM if kind == _lib.SPMV_AUTO else kind
 
     def get_precision(self):
         out = np.zeros(max(self.tilenum, 1), np.uint8)

@@ -191,21 +191,21 @@ def test_policy_labels_match_restatement(mf, policy):
 
 
 @pytest.mark.parametrize("name", ["lap3d_scaled32", "knn", "powerlaw"])
-@pytest.mark.parametrize("resident", [True, False])
-def test_learned_cg_reaches_tolerance(mf, oracle, policy, small_params, name, resident, monkeypatch):
-    if not resident:
-        monkeypatch.setenv("MF_NO_RESIDENT", "1")
-    print(f"case learned cg {name} resident={resident}", flush=True)
+@pytest.mark.parametrize("form", ["resident", "memory"])
+def test_learned_cg_reaches_tolerance(mf, oracle, policy, small_params, name, form):
+    print(f"case learned cg {name} form={form}", flush=True)
     rp, ci, va, b = _system(name)
     n = len(rp) - 1
     tm = mf.TiledMatrix(rp, ci, va)
     policy.install(tm, small_params)
     db, dx = tm.vector(b), tm.vector()
-    rep = tm.cg(db, dx, tol=TOL, policy="learned")
+    rep = tm.cg(db, dx, tol=TOL, policy="learned", form=form)
     rel = oracle.rel_residual(rp, ci, va, b, dx.get()[:n])
     assert rep["status"] == 0 and rel <= TOL, (rel, rep)
     assert rep["policy_used"] == mf.POLICY_LEARNED
     assert sum(rep["prec_hist"][1:4]) > 0, rep
+    if form == "memory" or name == "lap3d_scaled32":  # the stencil is regular: resident applies
+        assert rep["form_used"] == (2 if form == "resident" else 1), rep
 
 
 def test_learned_bicgstab_reaches_tolerance(mf, oracle, policy, small_params):

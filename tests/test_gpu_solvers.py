@@ -54,9 +54,10 @@ def _solve(mf, oracle, name, method="cg", **kw):
     return rep, rel, x, (rp, ci, va, b)
 
 
+@pytest.mark.parametrize("form", ["memory", "resident"])
 @pytest.mark.parametrize("name", ["lap2d64", "lap3d_scaled32", "lap2d30", "knn", "powerlaw"])
-def test_cg_fp64_reaches_tolerance(mf, oracle, name):
-    rep, rel, x, (rp, ci, va, b) = _solve(mf, oracle, name)
+def test_cg_fp64_reaches_tolerance(mf, oracle, name, form):
+    rep, rel, x, (rp, ci, va, b) = _solve(mf, oracle, name, form=form)
     assert rep["status"] == 0 and rel <= TOL, (rel, rep)
     assert abs(rep["rel_residual"] - rel) <= 1e-3 * TOL + 1e-15
     _, it_ref, rel_ref = oracle.cg(rp, ci, va, b, tol=TOL)
@@ -65,10 +66,11 @@ def test_cg_fp64_reaches_tolerance(mf, oracle, name):
         np.testing.assert_allclose(x, 1.0, atol=1e-7)
 
 
+@pytest.mark.parametrize("form", ["memory", "resident"])
 @pytest.mark.parametrize("policy", ["rule", "errbound"])
 @pytest.mark.parametrize("name", ["lap2d64", "lap3d_scaled32", "powerlaw"])
-def test_cg_mixed_policies_reach_tolerance(mf, oracle, name, policy):
-    rep, rel, _, _ = _solve(mf, oracle, name, policy=policy)
+def test_cg_mixed_policies_reach_tolerance(mf, oracle, name, policy, form):
+    rep, rel, _, _ = _solve(mf, oracle, name, policy=policy, form=form)
     assert rep["status"] == 0 and rel <= TOL, (rel, rep)
     assert sum(rep["prec_hist"]) > 0, rep
 
@@ -119,6 +121,22 @@ def test_solver_is_deterministic_and_takes_initial_guess(mf, oracle):
     rep = tm.cg(db, dx, tol=TOL)
     assert rep["iterations"] <= 1
     assert oracle.rel_residual(rp, ci, va, b, dx.get()[:n]) <= TOL
+
+
+@pytest.mark.parametrize("policy", ["fp64", "errbound"])
+def test_resident_form_is_deterministic(mf, oracle, policy):
+    rp, ci, va, b = _system("lap3d_scaled32")
+    n = len(rp) - 1
+    tm = mf.TiledMatrix(rp, ci, va)
+    db = tm.vector(b)
+    xs = []
+    for _ in range(2):
+        dx = tm.vector()
+        rep = tm.cg(db, dx, tol=TOL, policy=policy, form="resident")
+        assert rep["form_used"] == mf.FORM_RESIDENT and rep["status"] == 0, rep
+        xs.append(dx.get())
+    np.testing.assert_array_equal(xs[0], xs[1])
+    assert oracle.rel_residual(rp, ci, va, b, xs[0][:n]) <= TOL
 
 
 def test_fixed_iteration_mode(mf):
