@@ -358,11 +358,18 @@ mf_status mf_policy_labels(mf_tiled *t, const double *d_p, double theta,
 /* ---------------------------------------------------------------------- */
 
 /* Solves many independent SPD systems, each with nrhs right-hand sides, by
- * CG in one persistent launch captured in a hipGraph (FP32 / FP16 tiles,
- * FP64 scalars and residual refresh).  systems[i] are tiled handles on the
- * same device; d_b[i] / d_x[i] point to npad_i * nrhs doubles (column-major
- * by right-hand side).  per_system_ms (optional, host, nsys entries) receives
- * each system's solve time as completion offsets within the launch.
+ * CG in one persistent launch captured in a hipGraph: every block takes
+ * (system, right-hand side) items, largest system first, from a global queue
+ * and solves each with block-local reductions.  Tiles run at the handle's
+ * levels (opts->policy FP64, FIXED or ERRBOUND) with FP64 scalars, FP64 x and
+ * FP64 residual refresh / fallback as mf_cg_solve.  systems[i] are square
+ * tiled handles on one device; d_b[i] / d_x[i] point to npad_i * nrhs
+ * doubles (right-hand side j at offset j * npad_i, zero padded; d_x holds the
+ * initial guesses).  iterations and rel_residual (optional, host) receive
+ * nsys * nrhs entries (system-major); per_system_ms (optional, nsys entries)
+ * the time from a system's first item start to its last item end; total_ms
+ * the launch time.  Returns MF_NOT_CONVERGED if any item missed the
+ * tolerance.
  * No reference counterpart (the reference solves one system per process). */
 mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs,
                             const double **d_b, double **d_x,

@@ -1,0 +1,403 @@
+// mf_batch.hip -- many independent systems, each with several right-hand
+// sides, solved by CG in one persistent launch captured in a hipGraph.
+//
+// A work item is one (system, right-hand side) pair.  Items are ordered
+// largest system first (the longest-processing-time rule) and handed out by a
+// global queue: every block takes the next item, solves it with its 8 waves
+// and takes another, until the queue is empty.  An item's right-hand sides
+// sit next to each other in the queue, so the blocks solving them run at the
+// same time and share the system's tiles through L2 and the Infinity Cache.
+//
+// Inside a block the solve is the single-system memory-form CG: the 8 waves
+// own nnz-balanced tile-row ranges of the system, the SpMV is the streamed
+// wave SpMV, dot products are deterministic block sums and every phase ends
+// at a block barrier; there is no grid barrier, so blocks need not be
+// co-resident and the launch is an ordinary one.  Precision follows the
+// handle's current tile levels (FP64, or the FIXED / ERRBOUND levels set
+// before the launch) with FP64 scalars, FP64 x, periodic FP64 true-residual
+// recomputation and FP64 fallback on stagnation.
+//
+// Timing: each item records the realtime clock (100 MHz) at start and end;
+// a system's time is its last end minus its first start.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "mf_device.h"
+
+namespace mf {
+
+constexpr int kBatchBlock = 512;
+constexpr int kBatchWaves = kBatchBlock / 64;
+
+struct BatchSys {
+  TileView A;
+  const int *wave_tr;  // kBatchWaves + 1 tile-row bounds
+  const double *b;     // npad * nrhs, right-hand side j at j * npad
+  double *x;           // npad * nrhs
+  int npad;
+  int mixed;           // 1: per-tile levels (A.prec), 0: every tile FP64
+};
+
+struct BatchArgs {
+  const BatchSys *sys;
+  const int2 *items;  // (system, right-hand side), largest system first
+  int nitems;
+  int *head;          // work-queue head
+  double *work;       // per block: r, p, q (npad_max doubles each), p32
+  long long stride;   // doubles per block
+  int *iters;
+  double *relres;
+  unsigned long long *tstamp;  // per item: start, end
+  double tol2;
+  int maxiter;
+  double ratio2;
+  int max_refresh;
+  int refresh_every;
+};
+
+__device__ __forceinline__ unsigned long long realtime() { return __builtin_amdgcn_s_memrealtime(); }
+
+__global__ void k_batch_reset(int *head, int nitems, int *iters, double *relres) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *head = 0;
+  if (i < nitems) {
+    iters[i] = 0;
+    relres[i] = -1.0;
+  }
+}
+
+__global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
+  __shared__ double lds[kBatchWaves];
+  __shared__ int item_s;
+  __shared__ WaveLds wl[kBatchWaves];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  WaveLds &L = wl[wv];
+  double *r = a.work + (long long)blockIdx.x * a.stride;
+  PrecCount cnt;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(a.head, 1);
+    __syncthreads();
+    // item, system and wave range are wave-uniform: scalar registers
+    const int item = __builtin_amdgcn_readfirstlane(item_s);
+    __syncthreads();
+    if (item >= a.nitems) break;
+    const unsigned long long t_start = realtime();
+    const int sidx = __builtin_amdgcn_readfirstlane(a.items[item].x);
+    const int rhs = __builtin_amdgcn_readfirstlane(a.items[item].y);
+    const BatchSys &S = a.sys[sidx];
+    const int npad = S.npad;
+    const double *b = S.b + (long long)rhs * npad;
+    double *x = S.x + (long long)rhs * npad;
+    double *p = r + npad, *q = p + npad;
+    float *p32 = (float *)(q + npad);
+    const TileView &A = S.A;
+    const int tr0 = __builtin_amdgcn_readfirstlane(S.wave_tr[wv]);
+    const int tr1 = __builtin_amdgcn_readfirstlane(S.wave_tr[wv + 1]);
+    const int row0 = tr0 * 16, row1 = tr1 * 16;
+    // gathered padding of p beyond the tile-rows must be zero
+    for (int i = 16 * A.tilem + threadIdx.x; i < npad; i += kBatchBlock) {
+      p[i] = 0.0;
+      p32[i] = 0.0f;
+    }
+    // r = b - A x, p = r
+    double rr0 = 0.0, bb0 = 0.0;
+    wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+      [&](const int i, const double ax) {
+        const double bv = b[i], rv = bv - ax;
+        r[i] = rv;
+        p[i] = rv;
+        p32[i] = (float)rv;
+        rr0 += rv * rv;
+        bb0 += bv * bv;
+      });
+    double rr = block_sum(rr0, lds);
+    const double bb = block_sum(bb0, lds);
+    const double stop2 = a.tol2 * bb;
+    double rr_ref = rr;
+    int it = 0, it_ref = 0, refresh = 0;
+    int force64 = S.mixed ? 0 : 1;
+    const bool mixed = S.mixed != 0;
+    bool restart = false;
+    bool done = bb == 0.0 || rr <= stop2;
+    __syncthreads();
+    while (!done) {
+      if (restart) {
+        double tr0v = 0.0;
+        wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+          [&](const int i, const double ax) {
+            const double rv = b[i] - ax;
+            r[i] = rv;
+            p[i] = rv;
+            p32[i] = (float)rv;
+            tr0v += rv * rv;
+          });
+        rr = block_sum(tr0v, lds);
+        rr_ref = rr;
+        it_ref = it;
+        ++refresh;
+        restart = false;
+        if (rr <= stop2) break;
+        __syncthreads();
+      }
+      ++it;
+      double pq0 = 0.0;
+      wave_spmv<kPerTile, false>(A, tr0, tr1, p, p32, nullptr, force64, L, lane, &cnt,
+        [&](const int i, const double qv) {
+          q[i] = qv;
+          pq0 += p[i] * qv;
+        });
+      const double pq = block_sum(pq0, lds);
+      if (mixed && !force64 && !(pq > 0.0 && isfinite(pq))) {
+        force64 = 1;
+        restart = true;
+        continue;
+      }
+      if (!(pq != 0.0) || !isfinite(pq)) break;
+      const double alpha = rr / pq;
+      double rn0 = 0.0;
+      for (int i = row0 + lane; i < row1; i += 64) {
+        x[i] += alpha * p[i];
+        const double rv = r[i] - alpha * q[i];
+        r[i] = rv;
+        rn0 += rv * rv;
+      }
+      double rr_new = block_sum(rn0, lds);
+      const bool converged = rr_new <= stop2;
+      const bool periodic = mixed && !force64 &&
+                            (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
+      if (converged || periodic) {
+        double tv = 0.0;
+        wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+          [&](const int i, const double ax) {
+            const double rv = b[i] - ax;
+            r[i] = rv;
+            tv += rv * rv;
+          });
+        const double tr = block_sum(tv, lds);
+        ++refresh;
+        it_ref = it;
+        if (tr <= stop2) { rr = tr; break; }
+        if (mixed && !force64 && (tr > 0.25 * rr_ref || refresh > a.max_refresh)) {
+          force64 = 1;
+          restart = true;
+        }
+        if (tr > 4.0 * rr_new) restart = true;
+        rr_new = tr;
+        rr_ref = tr;
+      }
+      if (it >= a.maxiter) { rr = rr_new; break; }
+      if (restart) {
+        __syncthreads();
+        continue;
+      }
+      const double beta = rr_new / rr;
+      rr = rr_new;
+      for (int i = row0 + lane; i < row1; i += 64) {
+        const double pv = r[i] + beta * p[i];
+        p[i] = pv;
+        p32[i] = (float)pv;
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+    // FP64 relative residual of the result
+    double fv = 0.0;
+    wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+      [&](const int i, const double ax) {
+        const double d = b[i] - ax;
+        fv += d * d;
+      });
+    const double fin = block_sum(fv, lds);
+    if (threadIdx.x == 0) {
+      a.iters[item] = it;
+      a.relres[item] = bb > 0.0 ? sqrt(fin / bb) : sqrt(fin);
+      a.tstamp[2 * item] = t_start;
+      a.tstamp[2 * item + 1] = realtime();
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace mf
+
+using namespace mf;
+
+extern "C" mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs, const double **d_b,
+                                       double **d_x, const mf_solve_opts *o, int *iterations,
+                                       double *rel_residual, double *per_system_ms,
+                                       double *total_ms) {
+  if (!systems || nsys <= 0 || nrhs <= 0 || !d_b || !d_x)
+    return fail(MF_ERR_ARG, "mf_batch_cg_solve: bad argument");
+  mf_solve_opts opt;
+  mf_solve_opts_default(&opt);
+  if (o) opt = *o;
+  if (opt.maxiter <= 0) opt.maxiter = 10000;
+  if (!(opt.refresh_ratio > 0.0) || opt.refresh_ratio >= 1.0) opt.refresh_ratio = 0.1;
+  if (opt.max_refresh <= 0) opt.max_refresh = 64;
+  if (opt.refresh_every <= 0) opt.refresh_every = 64;
+  if (!(opt.tol > 0.0)) opt.tol = 1e-10;
+  if (opt.policy != MF_POLICY_FP64 && opt.policy != MF_POLICY_FIXED && opt.policy != MF_POLICY_ERRBOUND)
+    return fail(MF_ERR_UNSUPPORTED, "mf_batch_cg_solve: policies FP64, FIXED and ERRBOUND only");
+  const int dev = systems[0] ? systems[0]->dev.device : -1;
+  for (int i = 0; i < nsys; ++i) {
+    if (!systems[i] || systems[i]->dev.device != dev || dev < 0 || !d_b[i] || !d_x[i])
+      return fail(MF_ERR_ARG, "mf_batch_cg_solve: systems must be device handles on one device");
+    if (systems[i]->tilem != systems[i]->tilen)
+      return fail(MF_ERR_ARG, "mf_batch_cg_solve: square systems only");
+  }
+  MF_HIP(hipSetDevice(dev));
+  hipStream_t st = (hipStream_t)opt.stream;
+  hipStream_t own = nullptr;
+  if (!st) {
+    MF_HIP(hipStreamCreate(&own));  // capture needs a non-null stream
+    st = own;
+  }
+  struct Owned {
+    std::vector<void *> p;
+    hipStream_t s = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Owned() {
+      if (ge) (void)hipGraphExecDestroy(ge);
+      if (g) (void)hipGraphDestroy(g);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      for (void *q : p)
+        if (q) (void)hipFree(q);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } own_res;
+  own_res.s = own;
+  auto dalloc = [&](void **q, size_t bytes) -> mf_status {
+    MF_HIP(hipMalloc(q, bytes));
+    own_res.p.push_back(*q);
+    return MF_OK;
+  };
+
+  // per-system levels, schedules and descriptors
+  std::vector<BatchSys> hs(nsys);
+  std::vector<int> htr((size_t)nsys * (kBatchWaves + 1));
+  int npad_max = 16;
+  for (int i = 0; i < nsys; ++i) {
+    mf_tiled *t = systems[i];
+    if (opt.policy == MF_POLICY_FIXED)
+      MF_HIP(hipMemcpyAsync(t->dev.prec, t->dev.prec_user, t->tilenum, hipMemcpyDeviceToDevice, st));
+    if (opt.policy == MF_POLICY_ERRBOUND) MF_TRY(apply_errbound_levels(t, st));
+    const std::vector<int> w = balance_tilerows(t, kBatchWaves);
+    std::copy(w.begin(), w.end(), htr.begin() + (size_t)i * (kBatchWaves + 1));
+    npad_max = std::max(npad_max, t->dev.npad);
+  }
+  int *d_tr = nullptr;
+  MF_TRY(dalloc((void **)&d_tr, sizeof(int) * htr.size()));
+  MF_HIP(hipMemcpyAsync(d_tr, htr.data(), sizeof(int) * htr.size(), hipMemcpyHostToDevice, st));
+  for (int i = 0; i < nsys; ++i) {
+    hs[i].A = view_of(systems[i]);
+    hs[i].wave_tr = d_tr + (size_t)i * (kBatchWaves + 1);
+    hs[i].b = d_b[i];
+    hs[i].x = d_x[i];
+    hs[i].npad = systems[i]->dev.npad;
+    hs[i].mixed = opt.policy == MF_POLICY_FP64 ? 0 : 1;
+  }
+  // items, largest system first
+  std::vector<int> order(nsys);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int u, int v) { return systems[u]->csrsize > systems[v]->csrsize; });
+  const int nitems = nsys * nrhs;
+  std::vector<int2> hitems;
+  hitems.reserve(nitems);
+  for (int s : order)
+    for (int j = 0; j < nrhs; ++j) hitems.push_back(make_int2(s, j));
+
+  int ncu = 0, nb = 0;
+  MF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_cg, kBatchBlock, 0));
+  if (nb < 1) nb = 1;
+  int G = std::min(nitems, ncu * nb);
+  if (opt.blocks > 0) G = std::min(G, opt.blocks);
+  // p32 needs npad_max floats: round the per-block stride up to whole doubles
+  const long long stride = 3LL * npad_max + (npad_max + 1) / 2 + 32;
+
+  BatchSys *d_sys = nullptr;
+  int2 *d_items = nullptr;
+  int *d_head = nullptr, *d_iters = nullptr;
+  double *d_rel = nullptr, *d_work = nullptr;
+  unsigned long long *d_ts = nullptr;
+  MF_TRY(dalloc((void **)&d_sys, sizeof(BatchSys) * nsys));
+  MF_TRY(dalloc((void **)&d_items, sizeof(int2) * nitems));
+  MF_TRY(dalloc((void **)&d_head, sizeof(int)));
+  MF_TRY(dalloc((void **)&d_iters, sizeof(int) * nitems));
+  MF_TRY(dalloc((void **)&d_rel, sizeof(double) * nitems));
+  MF_TRY(dalloc((void **)&d_ts, sizeof(unsigned long long) * 2 * nitems));
+  MF_TRY(dalloc((void **)&d_work, sizeof(double) * stride * G));
+  MF_HIP(hipMemcpyAsync(d_sys, hs.data(), sizeof(BatchSys) * nsys, hipMemcpyHostToDevice, st));
+  MF_HIP(hipMemcpyAsync(d_items, hitems.data(), sizeof(int2) * nitems, hipMemcpyHostToDevice, st));
+  MF_HIP(hipStreamSynchronize(st));
+
+  BatchArgs args;
+  args.sys = d_sys;
+  args.items = d_items;
+  args.nitems = nitems;
+  args.head = d_head;
+  args.work = d_work;
+  args.stride = stride;
+  args.iters = d_iters;
+  args.relres = d_rel;
+  args.tstamp = d_ts;
+  args.tol2 = opt.tol * opt.tol;
+  args.maxiter = opt.maxiter;
+  args.ratio2 = opt.refresh_ratio * opt.refresh_ratio;
+  args.max_refresh = opt.max_refresh;
+  args.refresh_every = opt.refresh_every;
+
+  // capture: queue reset (a kernel, not a memset node) + the persistent solve
+  MF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  k_batch_reset<<<(nitems + 255) / 256, 256, 0, st>>>(d_head, nitems, d_iters, d_rel);
+  k_batch_cg<<<G, kBatchBlock, 0, st>>>(args);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(st, &g);
+  own_res.g = g;
+  if (ce != hipSuccess) return fail(MF_ERR_HIP, std::string("mf_batch_cg_solve capture: ") + hipGetErrorString(ce));
+  MF_HIP(hipGraphInstantiate(&own_res.ge, g, nullptr, nullptr, 0));
+  MF_HIP(hipEventCreate(&own_res.e0));
+  MF_HIP(hipEventCreate(&own_res.e1));
+  MF_HIP(hipEventRecord(own_res.e0, st));
+  MF_HIP(hipGraphLaunch(own_res.ge, st));
+  MF_HIP(hipEventRecord(own_res.e1, st));
+  MF_HIP(hipEventSynchronize(own_res.e1));
+  float ms = 0.0f;
+  MF_HIP(hipEventElapsedTime(&ms, own_res.e0, own_res.e1));
+  if (total_ms) *total_ms = ms;
+
+  std::vector<int> hi(nitems);
+  std::vector<double> hr(nitems);
+  std::vector<unsigned long long> ht(2 * (size_t)nitems);
+  MF_HIP(hipMemcpy(hi.data(), d_iters, sizeof(int) * nitems, hipMemcpyDeviceToHost));
+  MF_HIP(hipMemcpy(hr.data(), d_rel, sizeof(double) * nitems, hipMemcpyDeviceToHost));
+  MF_HIP(hipMemcpy(ht.data(), d_ts, sizeof(unsigned long long) * 2 * nitems, hipMemcpyDeviceToHost));
+  bool all = true;
+  for (int k = 0; k < nitems; ++k) {
+    const int s = hitems[k].x, j = hitems[k].y;
+    if (iterations) iterations[(size_t)s * nrhs + j] = hi[k];
+    if (rel_residual) rel_residual[(size_t)s * nrhs + j] = hr[k];
+    if (!(hr[k] >= 0.0 && hr[k] <= opt.tol)) all = false;
+  }
+  if (per_system_ms) {
+    for (int s = 0; s < nsys; ++s) {
+      unsigned long long t0 = ~0ull, t1 = 0;
+      for (int k = 0; k < nitems; ++k)
+        if (hitems[k].x == s) {
+          t0 = std::min(t0, ht[2 * (size_t)k]);
+          t1 = std::max(t1, ht[2 * (size_t)k + 1]);
+        }
+      per_system_ms[s] = t1 > t0 ? (double)(t1 - t0) * 1e-5 : 0.0;  // 100 MHz ticks
+    }
+  }
+  if (!all) return fail(MF_NOT_CONVERGED, "mf_batch_cg_solve: some systems did not reach the tolerance");
+  return MF_OK;
+}

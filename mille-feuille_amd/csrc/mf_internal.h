@@ -91,4 +91,7 @@ int default_spmv_kernel(const mf_tiled *t);
 // No tile-row holds more than 4x the mean tile count + 8 (the row-group
 // kernel, one tile-row per 16 lanes, wastes little on such matrices).
 bool regular_tilerows(const mf_tiled *t);
+// Error-bound precision levels into the handle's current levels
+// (stream-ordered; stream is a hipStream_t).
+mf_status apply_errbound_levels(mf_tiled *t, void *stream);
 }  // namespace mf

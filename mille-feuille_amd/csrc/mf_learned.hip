@@ -628,8 +628,3 @@ extern "C" mf_status mf_policy_labels(mf_tiled *t, const double *d_p, double the
   if (e != hipSuccess) return fail(MF_ERR_HIP, std::string("mf_policy_labels: ") + hipGetErrorString(e));
   return MF_OK;
 }
-
-extern "C" mf_status mf_batch_cg_solve(mf_tiled **, int, int, const double **, double **,
-                                       const mf_solve_opts *, int *, double *, double *, double *) {
-  return fail(MF_ERR_UNSUPPORTED, "mf_batch_cg_solve: not built yet");
-}

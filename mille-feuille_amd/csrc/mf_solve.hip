@@ -814,6 +814,16 @@ __global__ void k_errbound_levels(int T, const double *__restrict__ qerr,
 // Host side
 // ------------------------------------------------------------------------
 
+// Error-bound levels of every tile of t into t->dev.prec (stream-ordered).
+mf_status apply_errbound_levels(mf_tiled *t, void *stream) {
+  if (t->tilenum > 0) {
+    k_errbound_levels<<<(t->tilenum + 255) / 256, 256, 0, (hipStream_t)stream>>>(
+        t->tilenum, t->dev.qerr, t->dev.amax, t->dev.prec, 0x1p-24);
+    MF_HIP(hipGetLastError());
+  }
+  return MF_OK;
+}
+
 namespace {
 
 struct Workspace {

@@ -127,10 +127,8 @@ SIGNATURES = {
     "mf_policy_set_mlp": (C.c_int, [_vp, _vp]),
     "mf_policy_features": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp]),
     "mf_policy_labels": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp]),
-    "mf_batch_cg_solve": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.POINTER(_vp),
-                                    C.POINTER(_vp), C.POINTER(MfSolveOpts), _ip,
-                                    C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                    C.POINTER(C.c_double)]),
+    "mf_batch_cg_solve": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(MfSolveOpts), _vp,
+                                    _vp, _vp, C.POINTER(C.c_double)]),
     "mf_last_error": (C.c_char_p, []),
     "mf_version": (C.c_char_p, []),
 }

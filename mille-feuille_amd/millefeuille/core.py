@@ -229,3 +229,38 @@ def spmv_bytes(tm: "TiledMatrix", levels=None, x_bytes=8):
     active = int(np.sum(lv != 4))
     n = tm.rowA
     return payload + 24 * active + 4 * scaled + n * x_bytes + 8 * n
+
+
+def batch_cg(systems, bs, nrhs=1, x0=None, allow_not_converged=False, **kw):
+    """Solves every system in `systems` (TiledMatrix handles on one device)
+    for `nrhs` right-hand sides each, in one persistent launch captured in a
+    hipGraph (mf_batch_cg_solve).  bs[i] is an (nrhs, rowA_i) or (rowA_i,)
+    array.  Returns (xs, report) with xs[i] of shape (nrhs, rowA_i) and a
+    report holding iterations, relative residuals, per-system and total ms."""
+    opts = solve_opts(**kw)
+    nsys = len(systems)
+    db, dx = [], []
+    for tm, b in zip(systems, bs):
+        b = np.asarray(b, np.float64).reshape(nrhs, -1)
+        hb = np.zeros((nrhs, tm.npad))
+        hb[:, : b.shape[1]] = b
+        db.append(DeviceArray.from_host(hb.ravel()))
+        hx = np.zeros((nrhs, tm.npad))
+        if x0 is not None:
+            x = np.asarray(x0[len(dx)], np.float64).reshape(nrhs, -1)
+            hx[:, : x.shape[1]] = x
+        dx.append(DeviceArray.from_host(hx.ravel()))
+    hs = (C.c_void_p * nsys)(*[tm.h.value for tm in systems])
+    pb = (C.c_void_p * nsys)(*[d.ptr for d in db])
+    px = (C.c_void_p * nsys)(*[d.ptr for d in dx])
+    its = np.zeros(nsys * nrhs, np.int32)
+    rel = np.zeros(nsys * nrhs, np.float64)
+    per = np.zeros(nsys, np.float64)
+    tot = C.c_double()
+    code = lib.mf_batch_cg_solve(hs, nsys, nrhs, pb, px, C.byref(opts), _p(its), _p(rel), _p(per),
+                                 C.byref(tot))
+    check("mf_batch_cg_solve", code, allow=(0, 1) if allow_not_converged else (0,))
+    xs = [d.get().reshape(nrhs, tm.npad)[:, : tm.rowA] for d, tm in zip(dx, systems)]
+    return xs, {"status": code, "iterations": its.reshape(nsys, nrhs),
+                "rel_residual": rel.reshape(nsys, nrhs), "per_system_ms": per,
+                "total_ms": tot.value}

@@ -112,12 +112,24 @@ def uniform_vector(n, seed, lo=-1.0, hi=1.0):
     return np.random.default_rng(seed).uniform(lo, hi, n)
 
 
+def _morton_order(pts, bits=10):
+    """Indices that sort 3D points in [0,1)^3 along a Morton (Z-order) curve."""
+    q = np.minimum((pts * (1 << bits)).astype(np.int64), (1 << bits) - 1)
+    code = np.zeros(len(pts), np.int64)
+    for b in range(bits):
+        for d in range(3):
+            code |= ((q[:, d] >> b) & 1) << (3 * b + d)
+    return np.argsort(code, kind="stable")
+
+
 def knn_laplacian(n, k=8, seed=4, shift=1e-2):
     """Symmetrised kNN graph Laplacian on n uniform points in the unit cube,
-    L = D - W + shift*I with unit weights (SPD).  Uses a uniform grid of
-    cells for the neighbour search."""
+    L = D - W + shift*I with unit weights (SPD).  The points are numbered
+    along a Morton (Z-order) curve, so graph neighbours fall in nearby rows
+    and 16x16 tiles (a symmetric renumbering of the same graph)."""
     rng = np.random.default_rng(seed)
     pts = rng.random((n, 3))
+    pts = pts[_morton_order(pts)]
     from scipy.spatial import cKDTree  # generator-only dependency
 
     tree = cKDTree(pts)
