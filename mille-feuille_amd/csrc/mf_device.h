@@ -98,8 +98,6 @@ struct PrecCount {
 // of two) that are accumulated in FP64; SKIP tiles contribute nothing.
 // force64 != 0 treats every tile as FP64 (the solver's fallback).
 // ---------------------------------------------------------------------------
-constexpr int kWinTiles = 128;
-constexpr int kWinRows = 16;
 
 struct WaveLds {
   int toff[kWinTiles + 1];  // payload offset of each tile, relative to the window
@@ -130,157 +128,224 @@ __device__ __forceinline__ double entry_product(const TileView &A, int lvl, int 
   return 0.0;
 }
 
-// One lane's share of a slice: four consecutive entries k0..k0+3, their
-// (row,col) bytes and the raw payload words at the level of the tile that
-// holds the first valid entry (FP64: 8 words, FP32: 4, FP16: 2, INT8: 1).
-struct SliceBuf {
-  int k0;
-  int j;    // window-relative tile of the first valid entry, -1: none valid
-  int lvl;  // level of tile j (the level the payload words were loaded at)
+// One lane's share of a slice: the four consecutive entries k0..k0+3 of the
+// window.  The tile of every entry is found from the staged offsets alone
+// (before any payload arrives), so the payload is loaded at the right level
+// at once: as one vector load when the valid entries share one level (FP64:
+// 8 words, FP32: 4, FP16: 2, INT8: 1, packed), else entry by entry into
+// words 2u, 2u+1 ("split").  x is gathered once the (row,col) bytes are in.
+struct Chunk {
   uint32_t rc4;
-  uint4 a, b;  // raw payload words 0-3 and 4-7
+  uint32_t jp;    // window-relative tile of entry u in byte u
+  int vmask;      // bit u: entry u lies in the window; 0: no load issued
+  int lvl;        // common level of the valid entries; -1: split
+  uint4 a, b;     // raw payload words 0-3 and 4-7
+  double x0, x1, x2, x3;  // x (FP64 entries) or x32 (other levels) at each entry's column
 };
 
-// Word i of the payload (i is a compile-time constant after unrolling).
-__device__ __forceinline__ uint32_t slice_word(const SliceBuf &s, int i) {
-  switch (i) {
-    case 0: return s.a.x;
-    case 1: return s.a.y;
-    case 2: return s.a.z;
-    case 3: return s.a.w;
-    case 4: return s.b.x;
-    case 5: return s.b.y;
-    case 6: return s.b.z;
-    default: return s.b.w;
+// x of entry u (u is a compile-time constant after unrolling; named members
+// rather than an array keep the values in registers).
+__device__ __forceinline__ double chunk_x(const Chunk &c, int u) {
+  switch (u) {
+    case 0: return c.x0;
+    case 1: return c.x1;
+    case 2: return c.x2;
+    default: return c.x3;
   }
 }
 
-template <int MODE>
-__device__ __forceinline__ void slice_load(const TileView &A, const WaveLds &L, int base,
-                                           int lane, int Z0, int Z1, int nt, bool all64,
-                                           SliceBuf &s) {
-  s.k0 = base + 4 * lane;
-  s.j = -1;
-  s.lvl = MF_PREC_FP64;
-  if (s.k0 + 3 < Z0 || s.k0 >= Z1) return;
-  s.rc4 = *(const uint32_t *)(A.rc + s.k0);
-  if (all64) {
-    const uint4 *q = (const uint4 *)(A.v64 + s.k0);
-    s.a = q[0];
-    s.b = q[1];
+__device__ __forceinline__ void chunk_set_x(Chunk &c, int u, double v) {
+  switch (u) {
+    case 0: c.x0 = v; break;
+    case 1: c.x1 = v; break;
+    case 2: c.x2 = v; break;
+    default: c.x3 = v; break;
   }
+}
+
+// Word i of the payload (i is a compile-time constant after unrolling).
+__device__ __forceinline__ uint32_t chunk_word(const Chunk &c, int i) {
+  switch (i) {
+    case 0: return c.a.x;
+    case 1: return c.a.y;
+    case 2: return c.a.z;
+    case 3: return c.a.w;
+    case 4: return c.b.x;
+    case 5: return c.b.y;
+    case 6: return c.b.z;
+    default: return c.b.w;
+  }
+}
+
+__device__ __forceinline__ void chunk_set_words(Chunk &c, int u, uint32_t lo, uint32_t hi) {
+  switch (u) {
+    case 0: c.a.x = lo; c.a.y = hi; break;
+    case 1: c.a.z = lo; c.a.w = hi; break;
+    case 2: c.b.x = lo; c.b.y = hi; break;
+    default: c.b.z = lo; c.b.w = hi; break;
+  }
+}
+
+// Does the chunk of lane `lane` in the slice at `base` hold a window entry?
+__device__ __forceinline__ bool chunk_live(int base, int lane, int Z0, int Z1) {
+  const int k0 = base + 4 * lane;
+  return k0 + 3 >= Z0 && k0 < Z1;
+}
+
+// Stage 1a (FP64 payload): the (row,col) bytes and the payload, which need no
+// window metadata, so they can be issued before the window is staged.
+__device__ __forceinline__ void chunk_load64(const TileView &A, int base, int lane, int Z0,
+                                             int Z1, Chunk &c) {
+  if (!chunk_live(base, lane, Z0, Z1)) return;
+  const int k0 = base + 4 * lane;
+  c.rc4 = *(const uint32_t *)(A.rc + k0);
+  const uint4 *q = (const uint4 *)(A.v64 + k0);
+  c.a = q[0];
+  c.b = q[1];
+}
+
+// Stage 1b: the tile of every entry, from the staged offsets alone.
+__device__ __forceinline__ void chunk_locate(const WaveLds &L, int base, int lane, int Z0,
+                                             int Z1, int nt, Chunk &c) {
+  c.vmask = 0;
+  c.lvl = MF_PREC_FP64;
+  if (!chunk_live(base, lane, Z0, Z1)) return;
+  const int k0 = base + 4 * lane;
   // tile of the first valid entry: binary search over the staged offsets
-  const int kf = (s.k0 > Z0 ? s.k0 : Z0) - Z0;
+  const int kf = (k0 > Z0 ? k0 : Z0) - Z0;
   int lo = 0, hi = nt - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (L.toff[mid] <= kf) lo = mid;
     else hi = mid - 1;
   }
-  s.j = lo;
-  if (!all64) {
-    s.lvl = L.tprec[lo];
-    if (s.lvl == MF_PREC_FP64) {
-      const uint4 *q = (const uint4 *)(A.v64 + s.k0);
-      s.a = q[0];
-      s.b = q[1];
-    } else if (s.lvl == MF_PREC_FP32) {
-      s.a = *(const uint4 *)(A.v32 + s.k0);
-    } else if (s.lvl == MF_PREC_FP16) {
-      const uint2 h = *(const uint2 *)(A.v16 + s.k0);
-      s.a.x = h.x;
-      s.a.y = h.y;
-    } else if (s.lvl == MF_PREC_INT8) {
-      s.a.x = *(const uint32_t *)(A.v8 + s.k0);
+  int j = lo;
+  uint32_t jp = 0;
+  int vm = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int k = k0 + u - Z0;
+    if (k >= 0 && k < Z1 - Z0) {
+      while (L.toff[j + 1] <= k) ++j;
+      vm |= 1 << u;
+    }
+    jp |= (uint32_t)j << (8 * u);
+  }
+  c.jp = jp;
+  c.vmask = vm;
+}
+
+// Stage 1a (per-tile levels): the (row,col) bytes, which need no metadata.
+__device__ __forceinline__ void chunk_load_rc(const TileView &A, int base, int lane, int Z0,
+                                              int Z1, Chunk &c) {
+  if (!chunk_live(base, lane, Z0, Z1)) return;
+  c.rc4 = *(const uint32_t *)(A.rc + base + 4 * lane);
+}
+
+// Stage 1c (per-tile levels, after chunk_locate): the payload at the level
+// of the chunk's tiles.
+__device__ __forceinline__ void chunk_load_levels(const TileView &A, const WaveLds &L, int base,
+                                                  int lane, Chunk &c) {
+  const int vm = c.vmask;
+  if (!vm) return;
+  const int k0 = base + 4 * lane;
+  const uint32_t jp = c.jp;
+  // common level of the valid entries
+  int lv = -2;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (vm >> u & 1) {
+      const int l = L.tprec[(jp >> (8 * u)) & 255];
+      lv = lv == -2 ? l : (lv == l ? lv : -1);
+    }
+  }
+  c.lvl = lv;
+  if (lv == MF_PREC_FP64) {
+    const uint4 *q = (const uint4 *)(A.v64 + k0);
+    c.a = q[0];
+    c.b = q[1];
+  } else if (lv == MF_PREC_FP32) {
+    c.a = *(const uint4 *)(A.v32 + k0);
+  } else if (lv == MF_PREC_FP16) {
+    const uint2 h = *(const uint2 *)(A.v16 + k0);
+    c.a.x = h.x;
+    c.a.y = h.y;
+  } else if (lv == MF_PREC_INT8) {
+    c.a.x = *(const uint32_t *)(A.v8 + k0);
+  } else if (lv == -1) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!(vm >> u & 1)) continue;
+      const int l = L.tprec[(jp >> (8 * u)) & 255];
+      const int k = k0 + u;
+      uint32_t w0 = 0, w1 = 0;
+      if (l == MF_PREC_FP64) {
+        const uint2 v = *(const uint2 *)(A.v64 + k);
+        w0 = v.x;
+        w1 = v.y;
+      } else if (l == MF_PREC_FP32) {
+        w0 = __builtin_bit_cast(uint32_t, A.v32[k]);
+      } else if (l == MF_PREC_FP16) {
+        w0 = A.v16[k];
+      } else if (l == MF_PREC_INT8) {
+        w0 = (uint32_t)(uint8_t)A.v8[k];
+      }
+      chunk_set_words(c, u, w0, w1);
     }
   }
 }
 
-template <int MODE>
-__device__ __forceinline__ void slice_consume(const TileView &A, WaveLds &L, const SliceBuf &s,
-                                              int Z0, int Z1, bool all64,
-                                              const double *__restrict__ x,
-                                              const float *__restrict__ x32) {
-  if (s.j < 0) return;
-  int j = s.j;
-  int key = -1;
-  double run = 0.0;
-  // Fast path: all four entries valid and inside tile j (the common case:
-  // tiles hold 12-18 entries on the stencil workloads).
-  if (s.k0 >= Z0 && s.k0 + 3 < Z1 && L.toff[j + 1] > s.k0 + 3 - Z0) {
-    const int xo = L.tcol[j];
-    const int ro = L.trow[j] * 16;
-    double prod[4];
-    const int lvl = all64 ? MF_PREC_FP64 : s.lvl;
-    if (lvl == MF_PREC_FP64) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        prod[u] = __builtin_bit_cast(double, ((uint64_t)slice_word(s, 2 * u + 1) << 32) |
-                                                 slice_word(s, 2 * u)) *
-                  x[xo + ((s.rc4 >> (8 * u)) & 15)];
-    } else if (lvl == MF_PREC_FP32) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        prod[u] = (double)(__builtin_bit_cast(float, slice_word(s, u)) *
-                           x32[xo + ((s.rc4 >> (8 * u)) & 15)]);
-    } else if (lvl == MF_PREC_FP16) {
-      const float sc = L.tscale[j];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint16_t h = (uint16_t)(slice_word(s, u >> 1) >> (16 * (u & 1)));
-        prod[u] = (double)(half_bits_to_float(h) * x32[xo + ((s.rc4 >> (8 * u)) & 15)] * sc);
-      }
-    } else if (lvl == MF_PREC_INT8) {
-      const float sc = L.tscale[j];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int8_t q = (int8_t)(s.a.x >> (8 * u));
-        prod[u] = (double)((float)q * x32[xo + ((s.rc4 >> (8 * u)) & 15)] * sc);
-      }
-    } else {
-      return;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int kk = ro + ((s.rc4 >> (8 * u + 4)) & 15);
-      if (kk != key) {
-        if (key >= 0) atomicAdd(&L.acc[key], run);
-        key = kk;
-        run = prod[u];
-      } else {
-        run += prod[u];
-      }
-    }
-    atomicAdd(&L.acc[key], run);
-    return;
-  }
+// Stage 2 (needs the (row,col) bytes): gather x at every valid entry's column.
+__device__ __forceinline__ void chunk_gather(const WaveLds &L, const double *__restrict__ x,
+                                             const float *__restrict__ x32, bool all64,
+                                             Chunk &c) {
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int k = s.k0 + u;
-    if (k < Z0 || k >= Z1) continue;
-    while (L.toff[j + 1] <= k - Z0) ++j;
-    const int rc = (s.rc4 >> (8 * u)) & 0xff;
-    const int xi = L.tcol[j] + (rc & 15);
-    const int kk = L.trow[j] * 16 + (rc >> 4);
+    if (!(c.vmask >> u & 1)) continue;
+    const int j = (c.jp >> (8 * u)) & 255;
+    const int xi = L.tcol[j] + ((c.rc4 >> (8 * u)) & 15);
+    if (all64) {
+      chunk_set_x(c, u, x[xi]);
+    } else {
+      const int l = L.tprec[j];
+      if (l == MF_PREC_FP64) chunk_set_x(c, u, x[xi]);
+      else if (l != MF_PREC_SKIP) chunk_set_x(c, u, (double)x32[xi]);
+    }
+  }
+}
+
+// Stage 3 (needs payload and x): products, merged per row in registers and
+// added into the window's LDS row accumulators.
+__device__ __forceinline__ void chunk_consume(WaveLds &L, const Chunk &c, bool all64) {
+  int key = -1;
+  double run = 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (!(c.vmask >> u & 1)) continue;
+    const int j = (c.jp >> (8 * u)) & 255;
+    const int kk = L.trow[j] * 16 + ((c.rc4 >> (8 * u + 4)) & 15);
     double prod;
     if (all64) {
-      prod = __builtin_bit_cast(double, ((uint64_t)slice_word(s, 2 * u + 1) << 32) | slice_word(s, 2 * u)) * x[xi];
+      prod = __builtin_bit_cast(double, ((uint64_t)chunk_word(c, 2 * u + 1) << 32) |
+                                            chunk_word(c, 2 * u)) * chunk_x(c, u);
     } else {
-      const int lvl = L.tprec[j];
-      if (lvl != s.lvl) {
-        prod = entry_product<MODE>(A, lvl, k, xi, x, x32, L.tscale[j]);
-      } else if (lvl == MF_PREC_FP64) {
-        prod = __builtin_bit_cast(double, ((uint64_t)slice_word(s, 2 * u + 1) << 32) | slice_word(s, 2 * u)) * x[xi];
-      } else if (lvl == MF_PREC_FP32) {
-        prod = (double)(__builtin_bit_cast(float, slice_word(s, u)) * x32[xi]);
-      } else if (lvl == MF_PREC_FP16) {
-        const uint16_t h = (uint16_t)(slice_word(s, u >> 1) >> (16 * (u & 1)));
-        prod = (double)(half_bits_to_float(h) * x32[xi] * L.tscale[j]);
-      } else if (lvl == MF_PREC_INT8) {
-        const int8_t q = (int8_t)(s.a.x >> (8 * u));
-        prod = (double)((float)q * x32[xi] * L.tscale[j]);
+      const int l = L.tprec[j];
+      const bool split = c.lvl < 0;
+      if (l == MF_PREC_FP64) {
+        prod = __builtin_bit_cast(double, ((uint64_t)chunk_word(c, 2 * u + 1) << 32) |
+                                              chunk_word(c, 2 * u)) * chunk_x(c, u);
+      } else if (l == MF_PREC_FP32) {
+        const uint32_t w = split ? chunk_word(c, 2 * u) : chunk_word(c, u);
+        prod = (double)(__builtin_bit_cast(float, w) * (float)chunk_x(c, u));
+      } else if (l == MF_PREC_FP16) {
+        const uint16_t h = split ? (uint16_t)chunk_word(c, 2 * u)
+                                 : (uint16_t)(chunk_word(c, u >> 1) >> (16 * (u & 1)));
+        prod = (double)(half_bits_to_float(h) * (float)chunk_x(c, u) * L.tscale[j]);
+      } else if (l == MF_PREC_INT8) {
+        const int8_t q = split ? (int8_t)chunk_word(c, 2 * u) : (int8_t)(c.a.x >> (8 * u));
+        prod = (double)((float)q * (float)chunk_x(c, u) * L.tscale[j]);
       } else {
-        prod = 0.0;
+        continue;
       }
     }
     if (kk != key) {
@@ -294,58 +359,149 @@ __device__ __forceinline__ void slice_consume(const TileView &A, WaveLds &L, con
   if (key >= 0) atomicAdd(&L.acc[key], run);
 }
 
+// Descriptor of tile-rows [tr0, tr1) from the tile pointers (callers without
+// a precomputed one).
+__device__ __forceinline__ WaveDesc wave_desc(const TileView &A, int tr0, int tr1) {
+  WaveDesc d;
+  d.tr0 = tr0;
+  d.tr1 = tr1;
+  d.ta = A.tile_ptr[tr0];
+  d.T1 = A.tile_ptr[tr1];
+  d.Z0 = A.tile_off[d.ta];
+  d.Z1 = A.tile_off[d.T1];
+  d.pad0 = d.pad1 = 0;
+  return d;
+}
+
 template <int MODE, bool COUNT, typename Epi>
-__device__ __forceinline__ void wave_spmv(const TileView &A, int tr0, int tr1,
+__device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &desc,
                                           const double *__restrict__ x,
                                           const float *__restrict__ x32,
                                           const uint8_t *__restrict__ seglev, int force64,
                                           WaveLds &L, int lane, PrecCount *cnt, Epi epi) {
+  static_assert(kWinTiles == 128, "window staging covers tiles lane and lane + 64");
+  const int tr0 = __builtin_amdgcn_readfirstlane(desc.tr0);
+  const int tr1 = __builtin_amdgcn_readfirstlane(desc.tr1);
   if (tr0 >= tr1) return;
+  const int T1 = __builtin_amdgcn_readfirstlane(desc.T1);
   const bool all64 = MODE == kAll64 || force64;
   for (int i = lane; i < kWinRows * 16; i += 64) L.acc[i] = 0.0;
-  const int T1 = A.tile_ptr[tr1];
-  int ta = A.tile_ptr[tr0];
+  int ta = __builtin_amdgcn_readfirstlane(desc.ta);
+  int Za = __builtin_amdgcn_readfirstlane(desc.Z0);
   int ra = tr0;
   while (ra < tr1) {
     const int rlim = ra + kWinRows < tr1 ? ra + kWinRows : tr1;
     int tb = ta + kWinTiles < T1 ? ta + kWinTiles : T1;
     if (rlim < tr1) {
-      const int tl = A.tile_ptr[rlim];
+      const int tl = __builtin_amdgcn_readfirstlane(A.tile_ptr[rlim]);
       if (tl < tb) tb = tl;
     }
     const int nt = tb - ta;
+    int Zb = Za;
     if (nt > 0) {
-      const int Z0 = A.tile_off[ta], Z1 = A.tile_off[tb];
-      for (int j = lane; j <= nt; j += 64) L.toff[j] = A.tile_off[ta + j] - Z0;
-      for (int j = lane; j < nt; j += 64) {
-        const int t = ta + j;
-        L.tcol[j] = A.tile_col[t] << 4;
-        L.trow[j] = (uint8_t)(A.tile_row[t] - ra);
+      const int Z0 = Za;
+      const int Z1 = tb == T1 ? __builtin_amdgcn_readfirstlane(desc.Z1)
+                              : __builtin_amdgcn_readfirstlane(A.tile_off[tb]);
+      Zb = Z1;
+      const int zb = Z0 & ~3;
+      const int nsl = (Z1 - zb + 255) >> 8;
+      // Window metadata into registers (tiles lane and lane + 64; offsets
+      // also of tile 128), then the first slice's loads that need none of
+      // it ((row,col) bytes; FP64 payload), then the metadata into LDS.
+      const int ja = lane, jb = lane + 64;
+      int oa = 0, ob = 0, oc = 0, ca = 0, cb = 0, wa = 0, wb = 0, la = 0, lb = 0;
+      float sa = 0.0f, sb = 0.0f;
+      if (ja <= nt) oa = A.tile_off[ta + ja];
+      if (jb <= nt) ob = A.tile_off[ta + jb];
+      if (lane == 0 && nt == kWinTiles) oc = A.tile_off[ta + kWinTiles];
+      if (ja < nt) {
+        ca = A.tile_col[ta + ja];
+        wa = A.tile_row[ta + ja];
         if (MODE != kAll64) {
-          const int lv = tile_level<MODE>(A, seglev, t, force64);
-          L.tprec[j] = (uint8_t)lv;
-          L.tscale[j] = lv == MF_PREC_FP16 ? A.s16[t] : A.s8[t];
-          if (COUNT) cnt->add(lv);
+          la = tile_level<MODE>(A, seglev, ta + ja, force64);
+          sa = la == MF_PREC_FP16 ? A.s16[ta + ja] : A.s8[ta + ja];
+        }
+      }
+      if (jb < nt) {
+        cb = A.tile_col[ta + jb];
+        wb = A.tile_row[ta + jb];
+        if (MODE != kAll64) {
+          lb = tile_level<MODE>(A, seglev, ta + jb, force64);
+          sb = lb == MF_PREC_FP16 ? A.s16[ta + jb] : A.s8[ta + jb];
+        }
+      }
+      Chunk cx, cy;
+      if (all64) chunk_load64(A, zb, lane, Z0, Z1, cx);
+      else chunk_load_rc(A, zb, lane, Z0, Z1, cx);
+      if (ja <= nt) L.toff[ja] = oa - Z0;
+      if (jb <= nt) L.toff[jb] = ob - Z0;
+      if (lane == 0 && nt == kWinTiles) L.toff[kWinTiles] = oc - Z0;
+      if (ja < nt) {
+        L.tcol[ja] = ca << 4;
+        L.trow[ja] = (uint8_t)(wa - ra);
+        if (MODE != kAll64) {
+          L.tprec[ja] = (uint8_t)la;
+          L.tscale[ja] = sa;
+          if (COUNT) cnt->add(la);
+        }
+      }
+      if (jb < nt) {
+        L.tcol[jb] = cb << 4;
+        L.trow[jb] = (uint8_t)(wb - ra);
+        if (MODE != kAll64) {
+          L.tprec[jb] = (uint8_t)lb;
+          L.tscale[jb] = sb;
+          if (COUNT) cnt->add(lb);
         }
       }
       __builtin_amdgcn_wave_barrier();
-      // Slices of 256 entries; the loads of the next slice are issued before
-      // the current one is consumed.
-      const int zb = Z0 & ~3;
-      const int nsl = (Z1 - zb + 255) >> 8;
-      SliceBuf cur, nxt;
-      slice_load<MODE>(A, L, zb, lane, Z0, Z1, nt, all64, cur);
-      for (int m = 0; m < nsl; ++m) {
-        if (m + 1 < nsl) slice_load<MODE>(A, L, zb + 256 * (m + 1), lane, Z0, Z1, nt, all64, nxt);
-        slice_consume<MODE>(A, L, cur, Z0, Z1, all64, x, x32);
-        cur = nxt;
+      // Slices of 256 entries, with the loads of slice m+1 in flight while
+      // slice m is gathered and consumed.  FP64 payload: slice m+1's loads
+      // need no metadata and go out right after slice m is located.  Per-tile
+      // levels: slice m+1 is located and its loads issued before slice m's
+      // gathers.  Two chunk buffers in fixed roles, so no register copy
+      // waits for loads in flight.
+      if (all64) {
+        for (int m = 0; m < nsl; m += 2) {
+          const int b0 = zb + 256 * m;
+          chunk_locate(L, b0, lane, Z0, Z1, nt, cx);
+          if (m + 1 < nsl) chunk_load64(A, b0 + 256, lane, Z0, Z1, cy);
+          chunk_gather(L, x, x32, true, cx);
+          chunk_consume(L, cx, true);
+          if (m + 1 >= nsl) break;
+          chunk_locate(L, b0 + 256, lane, Z0, Z1, nt, cy);
+          if (m + 2 < nsl) chunk_load64(A, b0 + 512, lane, Z0, Z1, cx);
+          chunk_gather(L, x, x32, true, cy);
+          chunk_consume(L, cy, true);
+        }
+      } else {
+        chunk_locate(L, zb, lane, Z0, Z1, nt, cx);
+        chunk_load_levels(A, L, zb, lane, cx);
+        for (int m = 0; m < nsl; m += 2) {
+          const int b1 = zb + 256 * (m + 1);
+          if (m + 1 < nsl) {
+            chunk_locate(L, b1, lane, Z0, Z1, nt, cy);
+            chunk_load_rc(A, b1, lane, Z0, Z1, cy);
+            chunk_load_levels(A, L, b1, lane, cy);
+          }
+          chunk_gather(L, x, x32, false, cx);
+          chunk_consume(L, cx, false);
+          if (m + 1 >= nsl) break;
+          if (m + 2 < nsl) {
+            chunk_locate(L, b1 + 256, lane, Z0, Z1, nt, cx);
+            chunk_load_rc(A, b1 + 256, lane, Z0, Z1, cx);
+            chunk_load_levels(A, L, b1 + 256, lane, cx);
+          }
+          chunk_gather(L, x, x32, false, cy);
+          chunk_consume(L, cy, false);
+        }
       }
       __builtin_amdgcn_wave_barrier();
     }
     // Tile-rows [ra, R) are complete; tile-row R has started iff tile_ptr[R] < tb.
     int R;
     if (tb < T1) {
-      R = A.tile_row[tb];
+      R = __builtin_amdgcn_readfirstlane(A.tile_row[tb]);
       if (R > rlim) R = rlim;
     } else {
       R = rlim;
@@ -361,7 +517,19 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, int tr0, int tr1,
     __builtin_amdgcn_wave_barrier();
     ra = R;
     ta = tb;
+    Za = Zb;
   }
+}
+
+// Tile-rows [tr0, tr1) (descriptor loaded from the tile pointers).
+template <int MODE, bool COUNT, typename Epi>
+__device__ __forceinline__ void wave_spmv(const TileView &A, int tr0, int tr1,
+                                          const double *__restrict__ x,
+                                          const float *__restrict__ x32,
+                                          const uint8_t *__restrict__ seglev, int force64,
+                                          WaveLds &L, int lane, PrecCount *cnt, Epi epi) {
+  if (tr0 >= tr1) return;
+  wave_spmv<MODE, COUNT>(A, wave_desc(A, tr0, tr1), x, x32, seglev, force64, L, lane, cnt, epi);
 }
 
 // ---------------------------------------------------------------------------

@@ -14,6 +14,18 @@ namespace mf {
 
 constexpr int kTile = 16;  // tile edge (reference BLOCK_SIZE, src/common.h:47-50)
 
+// Window of the streamed SpMV: at most kWinTiles tiles spanning at most
+// kWinRows tile-rows are staged in a wave's LDS at a time.
+constexpr int kWinTiles = 128;
+constexpr int kWinRows = 16;
+
+// Work of one wave of the stand-alone streamed SpMV: tile-rows [tr0, tr1),
+// their tiles [ta, T1) and payload entries [Z0, Z1), precomputed so that a
+// wave starts loading payload without first walking the tile pointers.
+struct WaveDesc {
+  int tr0, tr1, ta, T1, Z0, Z1, pad0, pad1;
+};
+
 // Thread-local error message (mf_last_error).
 void set_error(const std::string &msg);
 mf_status fail(mf_status code, const std::string &msg);
@@ -46,8 +58,8 @@ struct DeviceLayout {
   void *mlp = nullptr;
   uint16_t *feat16 = nullptr;
   int has_mlp = 0;
-  // stand-alone SpMV schedule: nnz-balanced tile-row ranges, one per wave
-  int *spmv_tr = nullptr;
+  // stand-alone SpMV schedule: one descriptor per wave (window_partition)
+  WaveDesc *spmv_desc = nullptr;
   int spmv_nwaves = 0;
   int spmv_wave_nnz = 0;
 };
@@ -86,6 +98,10 @@ mf_status upload_tiles(mf_tiled *t, int device);
 void free_device(mf_tiled *t);
 // nnz-balanced contiguous tile-row partition for `nwaves` waves.
 std::vector<int> balance_tilerows(const mf_tiled *t, int nwaves);
+// Greedy split into waves of about target_nnz entries that each fit one SpMV
+// window (<= kWinRows tile-rows, <= kWinTiles tiles) unless a single tile-row
+// is larger than a window (such a tile-row gets a wave of its own).
+std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz);
 // Default stand-alone SpMV kernel (MF_SPMV_STREAM or MF_SPMV_ROWGROUP).
 int default_spmv_kernel(const mf_tiled *t);
 // No tile-row holds more than 4x the mean tile count + 8 (the row-group

@@ -14,17 +14,17 @@
 namespace mf {
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_spmv(TileView A, const int *__restrict__ wave_tr,
+__global__ __launch_bounds__(256) void k_spmv(TileView A, const WaveDesc *__restrict__ desc,
                                               int nwaves, const double *__restrict__ x,
                                               const float *__restrict__ x32,
                                               double *__restrict__ y) {
   __shared__ WaveLds wl[4];
   const int per_xcd = gridDim.x >> 3;  // gridDim.x is a multiple of 8
   const int lb = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-  const int w = lb * 4 + (threadIdx.x >> 6);
+  const int w = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (w >= nwaves) return;
-  wave_spmv<MODE, false>(A, wave_tr[w], wave_tr[w + 1], x, x32, nullptr, 0, wl[threadIdx.x >> 6],
+  wave_spmv<MODE, false>(A, desc[w], x, x32, nullptr, 0, wl[threadIdx.x >> 6],
                          lane, nullptr, [&](const int i, const double v) { y[i] = v; });
 }
 
@@ -69,9 +69,9 @@ mf_status launch_spmv(const mf_tiled *t, const double *d_x, const float *d_x32, 
     const int blocks = ((nw + 3) / 4 + 7) / 8 * 8;
     if (A.tilem > 0 && nw > 0) {
       if (mode == kAll64)
-        k_spmv<kAll64><<<blocks, 256, 0, s>>>(A, t->dev.spmv_tr, nw, d_x, d_x32, d_y);
+        k_spmv<kAll64><<<blocks, 256, 0, s>>>(A, t->dev.spmv_desc, nw, d_x, d_x32, d_y);
       else
-        k_spmv<kPerTile><<<blocks, 256, 0, s>>>(A, t->dev.spmv_tr, nw, d_x, d_x32, d_y);
+        k_spmv<kPerTile><<<blocks, 256, 0, s>>>(A, t->dev.spmv_desc, nw, d_x, d_x32, d_y);
       MF_HIP(hipGetLastError());
     }
   }

@@ -15,8 +15,8 @@
 //   qerr      3T f64           per-tile max |a - dequant(a)| for FP32/16/INT8
 //   amax      T f64            per-tile max |a|
 //   prec      T u8             current precision level of each tile
-//   spmv_tr                    nnz-balanced tile-row ranges of the
-//                              stand-alone SpMV waves
+//   spmv_desc                  per-wave work (tile-rows, tiles, entries)
+//                              of the stand-alone SpMV
 // Entries keep the reference's order (tile by tile, row by row inside a
 // tile, src/csr2block.h:239-252); the per-entry row nibble replaces the
 // reference's 16 row pointers per tile on the device, so one SpMV reads
@@ -129,10 +129,10 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   }
   d.max_tiles_per_row = max_tiles;
   d.spmv_kernel = default_spmv_kernel(t);
-  // nnz-balanced tile-row ranges for the stand-alone SpMV, about
-  // kSpmvWaveNnz entries per wave (retuned by mf_tile_set_schedule).
-  const int nw = std::max(1, (int)((Z + kSpmvWaveNnz - 1) / kSpmvWaveNnz));
-  std::vector<int> wtr = balance_tilerows(t, nw);
+  // per-wave work of the stand-alone SpMV, about kSpmvWaveNnz entries per
+  // wave (retuned by mf_tile_set_schedule)
+  const std::vector<WaveDesc> wdesc = window_partition(t, kSpmvWaveNnz);
+  const int nw = (int)wdesc.size();
   d.spmv_wave_nnz = kSpmvWaveNnz;
   MF_TRY(dalloc(&d.tile_ptr, TM + 1));
   MF_TRY(dalloc(&d.tile_col, T));
@@ -144,7 +144,7 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_TRY(dalloc(&d.v32, (size_t)Z + kPad));
   MF_TRY(dalloc(&d.v16, (size_t)Z + kPad));
   MF_TRY(dalloc(&d.v8, (size_t)Z + kPad));
-  MF_TRY(dalloc(&d.spmv_tr, nw + 1));
+  if (nw > 0) MF_TRY(dalloc(&d.spmv_desc, nw));
   d.spmv_nwaves = nw;
   MF_TRY(dalloc(&d.s16, T));
   MF_TRY(dalloc(&d.s8, T));
@@ -164,7 +164,8 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_HIP(hipMemset(d.v16, 0, sizeof(uint16_t) * ((size_t)Z + kPad)));
   MF_HIP(hipMemset(d.v8, 0, (size_t)Z + kPad));
   MF_HIP(hipMemcpy(d.v64, t->Blockcsr_Val.data(), sizeof(double) * Z, hipMemcpyHostToDevice));
-  MF_HIP(hipMemcpy(d.spmv_tr, wtr.data(), sizeof(int) * (nw + 1), hipMemcpyHostToDevice));
+  if (nw > 0)
+    MF_HIP(hipMemcpy(d.spmv_desc, wdesc.data(), sizeof(WaveDesc) * nw, hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.diag_tile, diag.data(), sizeof(int) * TM, hipMemcpyHostToDevice));
   MF_HIP(hipMemset(d.prec, MF_PREC_FP64, T));
   MF_HIP(hipMemset(d.prec_user, MF_PREC_FP64, T));
@@ -196,7 +197,7 @@ int default_spmv_kernel(const mf_tiled *) {
 void free_device(mf_tiled *t) {
   DeviceLayout &d = t->dev;
   if (d.device >= 0) (void)hipSetDevice(d.device);
-  void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.rp16, d.spmv_tr,
+  void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.rp16, d.spmv_desc,
                   d.v64,      d.v32,      d.v16,      d.v8,       d.s16,    d.s8,
                   d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp, d.feat16,
                   t->sched.wave_tr, t->work};
@@ -294,18 +295,20 @@ extern "C" mf_status mf_tile_set_schedule(mf_tiled *t, int spmv_wave_nnz) {
   if (spmv_wave_nnz < 64 || spmv_wave_nnz > (1 << 20))
     return fail(MF_ERR_ARG, "mf_tile_set_schedule: spmv_wave_nnz out of [64, 2^20]");
   MF_HIP(hipSetDevice(t->dev.device));
-  const int nw = std::max(1, (int)(((int64_t)t->csrsize + spmv_wave_nnz - 1) / spmv_wave_nnz));
-  std::vector<int> wtr = balance_tilerows(t, nw);
-  int *p = nullptr;
-  MF_HIP(hipMalloc(&p, sizeof(int) * (nw + 1)));
-  hipError_t e = hipMemcpy(p, wtr.data(), sizeof(int) * (nw + 1), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    (void)hipFree(p);
-    return fail(MF_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  const std::vector<WaveDesc> wdesc = window_partition(t, spmv_wave_nnz);
+  const int nw = (int)wdesc.size();
+  WaveDesc *p = nullptr;
+  if (nw > 0) {
+    MF_HIP(hipMalloc(&p, sizeof(WaveDesc) * nw));
+    hipError_t e = hipMemcpy(p, wdesc.data(), sizeof(WaveDesc) * nw, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return fail(MF_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
   }
   MF_HIP(hipDeviceSynchronize());
-  if (t->dev.spmv_tr) MF_HIP(hipFree(t->dev.spmv_tr));
-  t->dev.spmv_tr = p;
+  if (t->dev.spmv_desc) MF_HIP(hipFree(t->dev.spmv_desc));
+  t->dev.spmv_desc = p;
   t->dev.spmv_nwaves = nw;
   t->dev.spmv_wave_nnz = spmv_wave_nnz;
   return MF_OK;
