@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
     }
     // r = b - A x, p = r
     double rr0 = 0.0, bb0 = 0.0;
-    wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+    wave_spmv<kAll64, false, 2>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
       [&](const int i, const double ax) {
         const double bv = b[i], rv = bv - ax;
         r[i] = rv;
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
     while (!done) {
       if (restart) {
         double tr0v = 0.0;
-        wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+        wave_spmv<kAll64, false, 2>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
           [&](const int i, const double ax) {
             const double rv = b[i] - ax;
             r[i] = rv;
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
       }
       ++it;
       double pq0 = 0.0;
-      wave_spmv<kPerTile, false>(A, tr0, tr1, p, p32, nullptr, force64, L, lane, &cnt,
+      wave_spmv<kPerTile, false, 2>(A, tr0, tr1, p, p32, nullptr, force64, L, lane, &cnt,
         [&](const int i, const double qv) {
           q[i] = qv;
           pq0 += p[i] * qv;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
                             (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
       if (converged || periodic) {
         double tv = 0.0;
-        wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+        wave_spmv<kAll64, false, 2>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
           [&](const int i, const double ax) {
             const double rv = b[i] - ax;
             r[i] = rv;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
     __syncthreads();
     // FP64 relative residual of the result
     double fv = 0.0;
-    wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+    wave_spmv<kAll64, false, 2>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
       [&](const int i, const double ax) {
         const double d = b[i] - ax;
         fv += d * d;

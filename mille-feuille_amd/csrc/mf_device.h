@@ -33,7 +33,8 @@ struct TileView {
   const int *tile_col;  // tilenum
   const int *tile_off;  // tilenum+1
   const int *tile_row;  // tilenum
-  const uint8_t *rc;    // nnz: (row << 4) | col inside the tile
+  const uint8_t *tile_nz;  // tilenum: nnz of the tile - 1
+  const uint8_t *rc;    // payload slots: (row << 4) | col inside the tile
   const uint8_t *rp16;  // 16 per tile: row starts inside the tile
   const double *v64;
   const float *v32;
@@ -49,7 +50,7 @@ struct TileView {
 
 inline TileView view_of(const mf_tiled *t) {
   const DeviceLayout &d = t->dev;
-  return TileView{d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc,   d.rp16,
+  return TileView{d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.tile_nz, d.rc, d.rp16,
                   d.v64,      d.v32,      d.v16,      d.v8,       d.s16,  d.s8,
                   d.prec,     t->tilem,   t->tilenum, d.npad};
 }
@@ -79,19 +80,20 @@ struct PrecCount {
 // Streamed tiled SpMV.
 //
 // A wave owns a contiguous range of tile-rows, i.e. a contiguous range of
-// tiles and of payload entries (tiles are stored tile-row by tile-row).  It
-// walks that range in windows of at most kWinTiles tiles spanning at most
-// kWinRows tile-rows.  Per window the tile metadata (payload offset, column,
-// tile-row, precision, scale) is staged in wave-private LDS, then the window's
-// entries are streamed: lane L takes the four consecutive entries
-// base+4L..base+4L+3, so one wave instruction covers 256 entries and the
-// values / (row,col) bytes arrive as 32-, 16- or 4-byte vector loads.  Each
-// lane finds the tile of its first entry by a binary search over the staged
-// offsets (overlapping the HBM loads), gathers x, and adds its products into
-// per-row FP64 accumulators in LDS, merging consecutive entries of one row in
-// registers first.  When a window ends, every tile-row it completed is
-// handed to the epilogue row by row (one lane per row); an unfinished last
-// tile-row keeps its accumulators for the next window.
+// tiles and of payload slots (tiles are stored tile-row by tile-row, each
+// tile's entries padded to a multiple of 4).  It walks that range in windows
+// of at most kWinTiles tiles, kWinRows tile-rows and kWinChunks 4-entry
+// chunks.  Per window the tile metadata (payload offset, column, tile-row,
+// entry count, precision, scale) and the tile of every chunk are staged in
+// wave-private LDS, then the window's chunks are streamed: lane L takes chunk
+// base/4 + L, so one wave instruction covers 256 payload slots and the
+// values / (row,col) bytes arrive as 32-, 16-, 8- or 4-byte vector loads.  A
+// chunk never straddles two tiles, so one lookup gives its tile, level,
+// column and row base; the lane gathers x, and adds its products into per-row
+// FP64 accumulators in LDS, merging consecutive entries of one row in
+// registers first.  When a window ends, every tile-row it completed is handed
+// to the epilogue row by row (one lane per row); an unfinished last tile-row
+// keeps its accumulators for the next window.
 //
 // Mixed precision: FP64 tiles multiply x; FP32 / FP16 / INT8 tiles multiply
 // the FP32 copy x32 with FP32 products (FP16/INT8 scaled by the tile's power
@@ -100,11 +102,12 @@ struct PrecCount {
 // ---------------------------------------------------------------------------
 
 struct WaveLds {
-  int toff[kWinTiles + 1];  // payload offset of each tile, relative to the window
   int tcol[kWinTiles];      // 16 * tile column
   float tscale[kWinTiles];  // FP16 / INT8 scale of the tile's current level
+  int tend[kWinTiles];      // end of the tile's real entries, relative to the window
   uint8_t trow[kWinTiles];  // tile-row relative to the window's first tile-row
   uint8_t tprec[kWinTiles];
+  uint8_t ctile[kWinChunks];  // window-relative tile of each 4-entry chunk
   double acc[kWinRows * 16];
 };
 
@@ -128,20 +131,33 @@ __device__ __forceinline__ double entry_product(const TileView &A, int lvl, int 
   return 0.0;
 }
 
-// One lane's share of a slice: the four consecutive entries k0..k0+3 of the
-// window.  The tile of every entry is found from the staged offsets alone
-// (before any payload arrives), so the payload is loaded at the right level
-// at once: as one vector load when the valid entries share one level (FP64:
-// 8 words, FP32: 4, FP16: 2, INT8: 1, packed), else entry by entry into
-// words 2u, 2u+1 ("split").  x is gathered once the (row,col) bytes are in.
+// One lane's chunk of a slice: the four payload slots k0..k0+3 of the window,
+// all in tile j (tiles start at multiples of 4); slots past the tile's last
+// entry are padding.  The payload is loaded at the tile's level (FP64: 8
+// words, FP32: 4, FP16: 2, INT8: 1, packed); x is gathered once the (row,col)
+// bytes are in.
 struct Chunk {
   uint32_t rc4;
-  uint32_t jp;    // window-relative tile of entry u in byte u
-  int vmask;      // bit u: entry u lies in the window; 0: no load issued
-  int lvl;        // common level of the valid entries; -1: split
+  int vmask;      // bit u: slot u holds an entry of the window; 0: none
+  int j;          // window-relative tile
+  int lvl;        // level of tile j
   uint4 a, b;     // raw payload words 0-3 and 4-7
-  double x0, x1, x2, x3;  // x (FP64 entries) or x32 (other levels) at each entry's column
+  double x0, x1, x2, x3;  // x (FP64 tiles) or x32 (other levels) at each entry's column
 };
+
+// Word i of the payload (i is a compile-time constant after unrolling).
+__device__ __forceinline__ uint32_t chunk_word(const Chunk &c, int i) {
+  switch (i) {
+    case 0: return c.a.x;
+    case 1: return c.a.y;
+    case 2: return c.a.z;
+    case 3: return c.a.w;
+    case 4: return c.b.x;
+    case 5: return c.b.y;
+    case 6: return c.b.z;
+    default: return c.b.w;
+  }
+}
 
 // x of entry u (u is a compile-time constant after unrolling; named members
 // rather than an array keep the values in registers).
@@ -163,190 +179,114 @@ __device__ __forceinline__ void chunk_set_x(Chunk &c, int u, double v) {
   }
 }
 
-// Word i of the payload (i is a compile-time constant after unrolling).
-__device__ __forceinline__ uint32_t chunk_word(const Chunk &c, int i) {
-  switch (i) {
-    case 0: return c.a.x;
-    case 1: return c.a.y;
-    case 2: return c.a.z;
-    case 3: return c.a.w;
-    case 4: return c.b.x;
-    case 5: return c.b.y;
-    case 6: return c.b.z;
-    default: return c.b.w;
+// Stage 1a: the (row,col) bytes, and the payload when every tile is FP64;
+// neither needs the window metadata.  c = chunk index within the window.
+__device__ __forceinline__ void chunk_load_rc(const TileView &A, int Z0, int c, int nch,
+                                              bool all64, Chunk &ch) {
+  if (c >= nch) return;
+  const int k0 = Z0 + 4 * c;
+  ch.rc4 = *(const uint32_t *)(A.rc + k0);
+  if (all64) {
+    const uint4 *q = (const uint4 *)(A.v64 + k0);
+    ch.a = q[0];
+    ch.b = q[1];
   }
 }
 
-__device__ __forceinline__ void chunk_set_words(Chunk &c, int u, uint32_t lo, uint32_t hi) {
-  switch (u) {
-    case 0: c.a.x = lo; c.a.y = hi; break;
-    case 1: c.a.z = lo; c.a.w = hi; break;
-    case 2: c.b.x = lo; c.b.y = hi; break;
-    default: c.b.z = lo; c.b.w = hi; break;
-  }
-}
-
-// Does the chunk of lane `lane` in the slice at `base` hold a window entry?
-__device__ __forceinline__ bool chunk_live(int base, int lane, int Z0, int Z1) {
-  const int k0 = base + 4 * lane;
-  return k0 + 3 >= Z0 && k0 < Z1;
-}
-
-// Stage 1a (FP64 payload): the (row,col) bytes and the payload, which need no
-// window metadata, so they can be issued before the window is staged.
-__device__ __forceinline__ void chunk_load64(const TileView &A, int base, int lane, int Z0,
-                                             int Z1, Chunk &c) {
-  if (!chunk_live(base, lane, Z0, Z1)) return;
-  const int k0 = base + 4 * lane;
-  c.rc4 = *(const uint32_t *)(A.rc + k0);
+// Stage 1a without a branch: a chunk past the window reloads the window's
+// last chunk (in bounds), so the number of loads in flight stays fixed.
+__device__ __forceinline__ void chunk_load64_fixed(const TileView &A, int Z0, int c, int nch,
+                                                   Chunk &ch) {
+  const int k0 = Z0 + 4 * (c < nch ? c : nch - 1);
+  ch.rc4 = *(const uint32_t *)(A.rc + k0);
   const uint4 *q = (const uint4 *)(A.v64 + k0);
-  c.a = q[0];
-  c.b = q[1];
+  ch.a = q[0];
+  ch.b = q[1];
 }
 
-// Stage 1b: the tile of every entry, from the staged offsets alone.
-__device__ __forceinline__ void chunk_locate(const WaveLds &L, int base, int lane, int Z0,
-                                             int Z1, int nt, Chunk &c) {
-  c.vmask = 0;
-  c.lvl = MF_PREC_FP64;
-  if (!chunk_live(base, lane, Z0, Z1)) return;
-  const int k0 = base + 4 * lane;
-  // tile of the first valid entry: binary search over the staged offsets
-  const int kf = (k0 > Z0 ? k0 : Z0) - Z0;
-  int lo = 0, hi = nt - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (L.toff[mid] <= kf) lo = mid;
-    else hi = mid - 1;
-  }
-  int j = lo;
-  uint32_t jp = 0;
-  int vm = 0;
+// Stage 2 without a branch (FP64 payload): x of all four slots; a slot past
+// the tile's entries gathers an in-bounds x of the tile's column block.
+__device__ __forceinline__ void chunk_gather64_fixed(const WaveLds &L, const double *__restrict__ x,
+                                                     Chunk &ch) {
+  const int xo = L.tcol[ch.j];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int k = k0 + u - Z0;
-    if (k >= 0 && k < Z1 - Z0) {
-      while (L.toff[j + 1] <= k) ++j;
-      vm |= 1 << u;
-    }
-    jp |= (uint32_t)j << (8 * u);
-  }
-  c.jp = jp;
-  c.vmask = vm;
+  for (int u = 0; u < 4; ++u) chunk_set_x(ch, u, x[xo + ((ch.rc4 >> (8 * u)) & 15)]);
 }
 
-// Stage 1a (per-tile levels): the (row,col) bytes, which need no metadata.
-__device__ __forceinline__ void chunk_load_rc(const TileView &A, int base, int lane, int Z0,
-                                              int Z1, Chunk &c) {
-  if (!chunk_live(base, lane, Z0, Z1)) return;
-  c.rc4 = *(const uint32_t *)(A.rc + base + 4 * lane);
+// Stage 1b: tile, level and valid slots of the chunk (LDS only).
+__device__ __forceinline__ void chunk_locate(const WaveLds &L, int c, int nch, bool all64,
+                                             Chunk &ch) {
+  ch.vmask = 0;
+  ch.j = 0;
+  ch.lvl = MF_PREC_FP64;
+  if (c >= nch) return;
+  const int j = L.ctile[c];
+  const int live = L.tend[j] - 4 * c;  // real entries from slot 0 on (>= 1)
+  ch.j = j;
+  ch.vmask = live >= 4 ? 15 : (1 << live) - 1;
+  if (!all64) ch.lvl = L.tprec[j];
 }
 
-// Stage 1c (per-tile levels, after chunk_locate): the payload at the level
-// of the chunk's tiles.
-__device__ __forceinline__ void chunk_load_levels(const TileView &A, const WaveLds &L, int base,
-                                                  int lane, Chunk &c) {
-  const int vm = c.vmask;
-  if (!vm) return;
-  const int k0 = base + 4 * lane;
-  const uint32_t jp = c.jp;
-  // common level of the valid entries
-  int lv = -2;
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    if (vm >> u & 1) {
-      const int l = L.tprec[(jp >> (8 * u)) & 255];
-      lv = lv == -2 ? l : (lv == l ? lv : -1);
-    }
-  }
-  c.lvl = lv;
+// Stage 1c (per-tile levels, after chunk_locate): the payload at the tile's level.
+__device__ __forceinline__ void chunk_load_payload(const TileView &A, int Z0, int c, Chunk &ch) {
+  if (!ch.vmask) return;
+  const int k0 = Z0 + 4 * c;
+  const int lv = ch.lvl;
   if (lv == MF_PREC_FP64) {
     const uint4 *q = (const uint4 *)(A.v64 + k0);
-    c.a = q[0];
-    c.b = q[1];
+    ch.a = q[0];
+    ch.b = q[1];
   } else if (lv == MF_PREC_FP32) {
-    c.a = *(const uint4 *)(A.v32 + k0);
+    ch.a = *(const uint4 *)(A.v32 + k0);
   } else if (lv == MF_PREC_FP16) {
     const uint2 h = *(const uint2 *)(A.v16 + k0);
-    c.a.x = h.x;
-    c.a.y = h.y;
+    ch.a.x = h.x;
+    ch.a.y = h.y;
   } else if (lv == MF_PREC_INT8) {
-    c.a.x = *(const uint32_t *)(A.v8 + k0);
-  } else if (lv == -1) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (!(vm >> u & 1)) continue;
-      const int l = L.tprec[(jp >> (8 * u)) & 255];
-      const int k = k0 + u;
-      uint32_t w0 = 0, w1 = 0;
-      if (l == MF_PREC_FP64) {
-        const uint2 v = *(const uint2 *)(A.v64 + k);
-        w0 = v.x;
-        w1 = v.y;
-      } else if (l == MF_PREC_FP32) {
-        w0 = __builtin_bit_cast(uint32_t, A.v32[k]);
-      } else if (l == MF_PREC_FP16) {
-        w0 = A.v16[k];
-      } else if (l == MF_PREC_INT8) {
-        w0 = (uint32_t)(uint8_t)A.v8[k];
-      }
-      chunk_set_words(c, u, w0, w1);
-    }
+    ch.a.x = *(const uint32_t *)(A.v8 + k0);
   }
 }
 
-// Stage 2 (needs the (row,col) bytes): gather x at every valid entry's column.
+// Stage 2 (needs the (row,col) bytes): gather x at every entry's column.
 __device__ __forceinline__ void chunk_gather(const WaveLds &L, const double *__restrict__ x,
                                              const float *__restrict__ x32, bool all64,
-                                             Chunk &c) {
+                                             Chunk &ch) {
+  if (!ch.vmask || ch.lvl == MF_PREC_SKIP) return;
+  const int xo = L.tcol[ch.j];
+  const bool wide = all64 || ch.lvl == MF_PREC_FP64;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    if (!(c.vmask >> u & 1)) continue;
-    const int j = (c.jp >> (8 * u)) & 255;
-    const int xi = L.tcol[j] + ((c.rc4 >> (8 * u)) & 15);
-    if (all64) {
-      chunk_set_x(c, u, x[xi]);
-    } else {
-      const int l = L.tprec[j];
-      if (l == MF_PREC_FP64) chunk_set_x(c, u, x[xi]);
-      else if (l != MF_PREC_SKIP) chunk_set_x(c, u, (double)x32[xi]);
-    }
+    if (!(ch.vmask >> u & 1)) continue;
+    const int xi = xo + ((ch.rc4 >> (8 * u)) & 15);
+    chunk_set_x(ch, u, wide ? x[xi] : (double)x32[xi]);
   }
 }
 
 // Stage 3 (needs payload and x): products, merged per row in registers and
 // added into the window's LDS row accumulators.
-__device__ __forceinline__ void chunk_consume(WaveLds &L, const Chunk &c, bool all64) {
+__device__ __forceinline__ void chunk_consume(WaveLds &L, const Chunk &ch, bool all64) {
+  if (!ch.vmask || ch.lvl == MF_PREC_SKIP) return;
+  const int ro = L.trow[ch.j] * 16;
+  const int lv = all64 ? MF_PREC_FP64 : ch.lvl;
+  const float sc = lv == MF_PREC_FP16 || lv == MF_PREC_INT8 ? L.tscale[ch.j] : 1.0f;
   int key = -1;
   double run = 0.0;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    if (!(c.vmask >> u & 1)) continue;
-    const int j = (c.jp >> (8 * u)) & 255;
-    const int kk = L.trow[j] * 16 + ((c.rc4 >> (8 * u + 4)) & 15);
+    if (!(ch.vmask >> u & 1)) continue;
+    const int kk = ro + ((ch.rc4 >> (8 * u + 4)) & 15);
     double prod;
-    if (all64) {
-      prod = __builtin_bit_cast(double, ((uint64_t)chunk_word(c, 2 * u + 1) << 32) |
-                                            chunk_word(c, 2 * u)) * chunk_x(c, u);
+    if (lv == MF_PREC_FP64) {
+      prod = __builtin_bit_cast(double, ((uint64_t)chunk_word(ch, 2 * u + 1) << 32) |
+                                            chunk_word(ch, 2 * u)) * chunk_x(ch, u);
+    } else if (lv == MF_PREC_FP32) {
+      prod = (double)(__builtin_bit_cast(float, chunk_word(ch, u)) * (float)chunk_x(ch, u));
+    } else if (lv == MF_PREC_FP16) {
+      const uint16_t h = (uint16_t)(chunk_word(ch, u >> 1) >> (16 * (u & 1)));
+      prod = (double)(half_bits_to_float(h) * (float)chunk_x(ch, u) * sc);
     } else {
-      const int l = L.tprec[j];
-      const bool split = c.lvl < 0;
-      if (l == MF_PREC_FP64) {
-        prod = __builtin_bit_cast(double, ((uint64_t)chunk_word(c, 2 * u + 1) << 32) |
-                                              chunk_word(c, 2 * u)) * chunk_x(c, u);
-      } else if (l == MF_PREC_FP32) {
-        const uint32_t w = split ? chunk_word(c, 2 * u) : chunk_word(c, u);
-        prod = (double)(__builtin_bit_cast(float, w) * (float)chunk_x(c, u));
-      } else if (l == MF_PREC_FP16) {
-        const uint16_t h = split ? (uint16_t)chunk_word(c, 2 * u)
-                                 : (uint16_t)(chunk_word(c, u >> 1) >> (16 * (u & 1)));
-        prod = (double)(half_bits_to_float(h) * (float)chunk_x(c, u) * L.tscale[j]);
-      } else if (l == MF_PREC_INT8) {
-        const int8_t q = split ? (int8_t)chunk_word(c, 2 * u) : (int8_t)(c.a.x >> (8 * u));
-        prod = (double)((float)q * (float)chunk_x(c, u) * L.tscale[j]);
-      } else {
-        continue;
-      }
+      const int8_t q = (int8_t)(ch.a.x >> (8 * u));
+      prod = (double)((float)q * (float)chunk_x(ch, u) * sc);
     }
     if (kk != key) {
       if (key >= 0) atomicAdd(&L.acc[key], run);
@@ -356,7 +296,7 @@ __device__ __forceinline__ void chunk_consume(WaveLds &L, const Chunk &c, bool a
       run += prod;
     }
   }
-  if (key >= 0) atomicAdd(&L.acc[key], run);
+  atomicAdd(&L.acc[key], run);
 }
 
 // Descriptor of tile-rows [tr0, tr1) from the tile pointers (callers without
@@ -373,7 +313,7 @@ __device__ __forceinline__ WaveDesc wave_desc(const TileView &A, int tr0, int tr
   return d;
 }
 
-template <int MODE, bool COUNT, typename Epi>
+template <int MODE, bool COUNT, int DEPTH = 3, typename Epi>
 __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &desc,
                                           const double *__restrict__ x,
                                           const float *__restrict__ x32,
@@ -396,27 +336,37 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
       const int tl = __builtin_amdgcn_readfirstlane(A.tile_ptr[rlim]);
       if (tl < tb) tb = tl;
     }
-    const int nt = tb - ta;
+    int nt = tb - ta;
     int Zb = Za;
     if (nt > 0) {
       const int Z0 = Za;
+      // Window metadata into registers: payload offsets of tiles lane,
+      // lane + 64 and 128 (one past the last tile when the window is full).
+      const int ja = lane, jb = lane + 64;
+      int oa = 0, ob = 0, oc = 0;
+      if (ja <= nt) oa = A.tile_off[ta + ja];
+      if (jb <= nt) ob = A.tile_off[ta + jb];
+      if (nt == kWinTiles) oc = A.tile_off[ta + kWinTiles];
+      // Keep the tiles whose payload ends within kWinChunks chunks (always
+      // at least one: a tile holds at most 64 chunks).
+      const int lim = Z0 + 4 * kWinChunks;
+      const int fit = __builtin_popcountll(__ballot(ja >= 1 && ja <= nt && oa <= lim)) +
+                      __builtin_popcountll(__ballot(jb <= nt && ob <= lim)) +
+                      (nt == kWinTiles && oc <= lim ? 1 : 0);
+      if (fit < nt) {
+        nt = fit;
+        tb = ta + nt;
+      }
       const int Z1 = tb == T1 ? __builtin_amdgcn_readfirstlane(desc.Z1)
                               : __builtin_amdgcn_readfirstlane(A.tile_off[tb]);
       Zb = Z1;
-      const int zb = Z0 & ~3;
-      const int nsl = (Z1 - zb + 255) >> 8;
-      // Window metadata into registers (tiles lane and lane + 64; offsets
-      // also of tile 128), then the first slice's loads that need none of
-      // it ((row,col) bytes; FP64 payload), then the metadata into LDS.
-      const int ja = lane, jb = lane + 64;
-      int oa = 0, ob = 0, oc = 0, ca = 0, cb = 0, wa = 0, wb = 0, la = 0, lb = 0;
+      const int nch = (Z1 - Z0) >> 2;
+      int ca = 0, cb = 0, wa = 0, wb = 0, na = 0, nb = 0, la = 0, lb = 0;
       float sa = 0.0f, sb = 0.0f;
-      if (ja <= nt) oa = A.tile_off[ta + ja];
-      if (jb <= nt) ob = A.tile_off[ta + jb];
-      if (lane == 0 && nt == kWinTiles) oc = A.tile_off[ta + kWinTiles];
       if (ja < nt) {
         ca = A.tile_col[ta + ja];
         wa = A.tile_row[ta + ja];
+        na = A.tile_nz[ta + ja];
         if (MODE != kAll64) {
           la = tile_level<MODE>(A, seglev, ta + ja, force64);
           sa = la == MF_PREC_FP16 ? A.s16[ta + ja] : A.s8[ta + ja];
@@ -425,20 +375,29 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
       if (jb < nt) {
         cb = A.tile_col[ta + jb];
         wb = A.tile_row[ta + jb];
+        nb = A.tile_nz[ta + jb];
         if (MODE != kAll64) {
           lb = tile_level<MODE>(A, seglev, ta + jb, force64);
           sb = lb == MF_PREC_FP16 ? A.s16[ta + jb] : A.s8[ta + jb];
         }
       }
+      // The first slice's loads that need no metadata go out now.
       Chunk cx, cy;
-      if (all64) chunk_load64(A, zb, lane, Z0, Z1, cx);
-      else chunk_load_rc(A, zb, lane, Z0, Z1, cx);
-      if (ja <= nt) L.toff[ja] = oa - Z0;
-      if (jb <= nt) L.toff[jb] = ob - Z0;
-      if (lane == 0 && nt == kWinTiles) L.toff[kWinTiles] = oc - Z0;
+      chunk_load_rc(A, Z0, lane, nch, all64, cx);
+      // Metadata into LDS; every chunk of tile j records j.  The payload
+      // end of tile j is the offset of tile j + 1 (shuffled in uniform
+      // control flow: lane + 1's oa, for lane 63 lane 0's ob; lane + 1's ob,
+      // for lane 63 oc).
+      const int na_next = __shfl(oa, (lane + 1) & 63);
+      const int nb_next = __shfl(ob, (lane + 1) & 63);
+      const int ob_first = __shfl(ob, 0);
       if (ja < nt) {
+        const int o = oa - Z0;
+        const int e = (lane < 63 ? na_next : ob_first) - Z0;
         L.tcol[ja] = ca << 4;
         L.trow[ja] = (uint8_t)(wa - ra);
+        L.tend[ja] = o + na + 1;
+        for (int q = o >> 2; q < (e >> 2); ++q) L.ctile[q] = (uint8_t)ja;
         if (MODE != kAll64) {
           L.tprec[ja] = (uint8_t)la;
           L.tscale[ja] = sa;
@@ -446,8 +405,12 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
         }
       }
       if (jb < nt) {
+        const int o = ob - Z0;
+        const int e = (lane < 63 ? nb_next : oc) - Z0;
         L.tcol[jb] = cb << 4;
         L.trow[jb] = (uint8_t)(wb - ra);
+        L.tend[jb] = o + nb + 1;
+        for (int q = o >> 2; q < (e >> 2); ++q) L.ctile[q] = (uint8_t)jb;
         if (MODE != kAll64) {
           L.tprec[jb] = (uint8_t)lb;
           L.tscale[jb] = sb;
@@ -455,42 +418,66 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
         }
       }
       __builtin_amdgcn_wave_barrier();
-      // Slices of 256 entries, with the loads of slice m+1 in flight while
+      // Slices of 64 chunks, with the loads of slice m+1 in flight while
       // slice m is gathered and consumed.  FP64 payload: slice m+1's loads
       // need no metadata and go out right after slice m is located.  Per-tile
       // levels: slice m+1 is located and its loads issued before slice m's
       // gathers.  Two chunk buffers in fixed roles, so no register copy
       // waits for loads in flight.
-      if (all64) {
+      const int nsl = (nch + 63) >> 6;
+      if (DEPTH == 3 && MODE == kAll64) {
+        // Three-deep (FP64 payload only): slice m+2's loads go out after
+        // slice m's gathers, so waiting for those gathers never waits for
+        // them.
+        Chunk cz;
+        if (nsl > 1) chunk_load64_fixed(A, Z0, lane + 64, nch, cy);
+        for (int m = 0; m < nsl; m += 3) {
+          const int c0 = 64 * m + lane;
+          chunk_locate(L, c0, nch, true, cx);
+          chunk_gather64_fixed(L, x, cx);
+          if (m + 2 < nsl) chunk_load64_fixed(A, Z0, c0 + 128, nch, cz);
+          chunk_consume(L, cx, true);
+          if (m + 1 >= nsl) break;
+          chunk_locate(L, c0 + 64, nch, true, cy);
+          chunk_gather64_fixed(L, x, cy);
+          if (m + 3 < nsl) chunk_load64_fixed(A, Z0, c0 + 192, nch, cx);
+          chunk_consume(L, cy, true);
+          if (m + 2 >= nsl) break;
+          chunk_locate(L, c0 + 128, nch, true, cz);
+          chunk_gather64_fixed(L, x, cz);
+          if (m + 4 < nsl) chunk_load64_fixed(A, Z0, c0 + 256, nch, cy);
+          chunk_consume(L, cz, true);
+        }
+      } else if (all64) {
         for (int m = 0; m < nsl; m += 2) {
-          const int b0 = zb + 256 * m;
-          chunk_locate(L, b0, lane, Z0, Z1, nt, cx);
-          if (m + 1 < nsl) chunk_load64(A, b0 + 256, lane, Z0, Z1, cy);
+          const int c0 = 64 * m + lane;
+          chunk_locate(L, c0, nch, true, cx);
+          if (m + 1 < nsl) chunk_load_rc(A, Z0, c0 + 64, nch, true, cy);
           chunk_gather(L, x, x32, true, cx);
           chunk_consume(L, cx, true);
           if (m + 1 >= nsl) break;
-          chunk_locate(L, b0 + 256, lane, Z0, Z1, nt, cy);
-          if (m + 2 < nsl) chunk_load64(A, b0 + 512, lane, Z0, Z1, cx);
+          chunk_locate(L, c0 + 64, nch, true, cy);
+          if (m + 2 < nsl) chunk_load_rc(A, Z0, c0 + 128, nch, true, cx);
           chunk_gather(L, x, x32, true, cy);
           chunk_consume(L, cy, true);
         }
       } else {
-        chunk_locate(L, zb, lane, Z0, Z1, nt, cx);
-        chunk_load_levels(A, L, zb, lane, cx);
+        chunk_locate(L, lane, nch, false, cx);
+        chunk_load_payload(A, Z0, lane, cx);
         for (int m = 0; m < nsl; m += 2) {
-          const int b1 = zb + 256 * (m + 1);
+          const int c1 = 64 * (m + 1) + lane;
           if (m + 1 < nsl) {
-            chunk_locate(L, b1, lane, Z0, Z1, nt, cy);
-            chunk_load_rc(A, b1, lane, Z0, Z1, cy);
-            chunk_load_levels(A, L, b1, lane, cy);
+            chunk_load_rc(A, Z0, c1, nch, false, cy);
+            chunk_locate(L, c1, nch, false, cy);
+            chunk_load_payload(A, Z0, c1, cy);
           }
           chunk_gather(L, x, x32, false, cx);
           chunk_consume(L, cx, false);
           if (m + 1 >= nsl) break;
           if (m + 2 < nsl) {
-            chunk_locate(L, b1 + 256, lane, Z0, Z1, nt, cx);
-            chunk_load_rc(A, b1 + 256, lane, Z0, Z1, cx);
-            chunk_load_levels(A, L, b1 + 256, lane, cx);
+            chunk_load_rc(A, Z0, c1 + 64, nch, false, cx);
+            chunk_locate(L, c1 + 64, nch, false, cx);
+            chunk_load_payload(A, Z0, c1 + 64, cx);
           }
           chunk_gather(L, x, x32, false, cy);
           chunk_consume(L, cy, false);
@@ -522,14 +509,15 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
 }
 
 // Tile-rows [tr0, tr1) (descriptor loaded from the tile pointers).
-template <int MODE, bool COUNT, typename Epi>
+template <int MODE, bool COUNT, int DEPTH = 3, typename Epi>
 __device__ __forceinline__ void wave_spmv(const TileView &A, int tr0, int tr1,
                                           const double *__restrict__ x,
                                           const float *__restrict__ x32,
                                           const uint8_t *__restrict__ seglev, int force64,
                                           WaveLds &L, int lane, PrecCount *cnt, Epi epi) {
   if (tr0 >= tr1) return;
-  wave_spmv<MODE, COUNT>(A, wave_desc(A, tr0, tr1), x, x32, seglev, force64, L, lane, cnt, epi);
+  wave_spmv<MODE, COUNT, DEPTH>(A, wave_desc(A, tr0, tr1), x, x32, seglev, force64, L, lane, cnt,
+                                epi);
 }
 
 // ---------------------------------------------------------------------------
@@ -581,7 +569,7 @@ __device__ __forceinline__ double rowgroup_row(const TileView &A, int R, bool va
       if (i + u < nt) {
         const int t = t0 + i + u;
         off[u] = A.tile_off[t];
-        const int nz = A.tile_off[t + 1] - off[u];
+        const int nz = A.tile_nz[t] + 1;
         xo[u] = A.tile_col[t] << 4;
         b[u] = A.rp16[(size_t)t * 16 + r];
         e[u] = r < 15 ? A.rp16[(size_t)t * 16 + r + 1] : nz;

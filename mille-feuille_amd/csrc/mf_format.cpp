@@ -182,7 +182,7 @@ std::vector<int> balance_tilerows(const mf_tiled *t, int nwaves) {
 std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz) {
   std::vector<WaveDesc> out;
   const int tm = t->tilem;
-  auto nnz_of = [&](int r) { return t->csr_offset[t->tile_ptr[r + 1]] - t->csr_offset[t->tile_ptr[r]]; };
+  auto nnz_of = [&](int r) { return t->pay_off[t->tile_ptr[r + 1]] - t->pay_off[t->tile_ptr[r]]; };
   int r = 0;
   while (r < tm) {
     const int r0 = r;
@@ -193,14 +193,14 @@ std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz) {
       nz += nnz_of(r);
       ++r;
     } while (r < tm && r - r0 < kWinRows && t->tile_ptr[r + 1] - t->tile_ptr[r0] <= kWinTiles &&
-             nz + nnz_of(r) <= target_nnz);
+             nz + nnz_of(r) <= target_nnz && nz + nnz_of(r) <= 4 * kWinChunks);
     WaveDesc d;
     d.tr0 = r0;
     d.tr1 = r;
     d.ta = t->tile_ptr[r0];
     d.T1 = t->tile_ptr[r];
-    d.Z0 = t->csr_offset[d.ta];
-    d.Z1 = t->csr_offset[d.T1];
+    d.Z0 = t->pay_off[d.ta];
+    d.Z1 = t->pay_off[d.T1];
     d.pad0 = d.pad1 = 0;
     out.push_back(d);
   }

@@ -15,9 +15,11 @@ namespace mf {
 constexpr int kTile = 16;  // tile edge (reference BLOCK_SIZE, src/common.h:47-50)
 
 // Window of the streamed SpMV: at most kWinTiles tiles spanning at most
-// kWinRows tile-rows are staged in a wave's LDS at a time.
+// kWinRows tile-rows and kWinChunks 4-entry chunks of payload are staged in
+// a wave's LDS at a time.
 constexpr int kWinTiles = 128;
 constexpr int kWinRows = 16;
+constexpr int kWinChunks = 1024;
 
 // Work of one wave of the stand-alone streamed SpMV: tile-rows [tr0, tr1),
 // their tiles [ta, T1) and payload entries [Z0, Z1), precomputed so that a
@@ -36,16 +38,18 @@ struct DeviceLayout {
   int npad = 0;                // padded vector length = 16 * max(tilem, tilen)
   int *tile_ptr = nullptr;     // tilem+1
   int *tile_col = nullptr;     // tilenum
-  int *tile_off = nullptr;     // tilenum+1 (payload offset; == csr_offset)
+  int *tile_off = nullptr;     // tilenum+1 payload offset of each tile (pay_off)
   int *tile_row = nullptr;     // tilenum (tile-row of each tile)
-  uint8_t *rc = nullptr;       // csrsize (+pad): (row << 4) | col inside the tile
+  uint8_t *tile_nz = nullptr;  // tilenum: entries of the tile - 1
+  int npay = 0;                // payload slots: entries padded per tile to 4
+  uint8_t *rc = nullptr;       // npay (+pad): (row << 4) | col inside the tile
   uint8_t *rp16 = nullptr;     // 16*tilenum: row starts inside each tile
   int max_tiles_per_row = 0;   // largest tile count of a tile-row
   int spmv_kernel = 0;         // MF_SPMV_STREAM or MF_SPMV_ROWGROUP
-  double *v64 = nullptr;       // csrsize FP64 payload
-  float *v32 = nullptr;        // csrsize FP32 payload
-  uint16_t *v16 = nullptr;     // csrsize FP16 payload (bits), scaled by s16
-  int8_t *v8 = nullptr;        // csrsize INT8 payload, scaled by s8
+  double *v64 = nullptr;       // npay FP64 payload
+  float *v32 = nullptr;        // npay FP32 payload
+  uint16_t *v16 = nullptr;     // npay FP16 payload (bits), scaled by s16
+  int8_t *v8 = nullptr;        // npay INT8 payload, scaled by s8
   float *s16 = nullptr;        // tilenum power-of-two scales
   float *s8 = nullptr;         // tilenum max/127 scales
   double *qerr = nullptr;      // 3*tilenum max |a - dequant_p(a)|, p=FP32,FP16,INT8
@@ -80,6 +84,9 @@ struct mf_tiled {
   std::vector<unsigned char> blknnznnz, Tile_csr_Col, Blockcsr_Ptr;
   std::vector<double> Blockcsr_Val;
   std::vector<float> Blockcsr_Val_Low;
+  // device payload offsets: each tile's entries start at a multiple of 4
+  // (entries padded per tile to a multiple of 4), tilenum+1
+  std::vector<int> pay_off;
   mf::DeviceLayout dev;
   mf::Schedule sched;
   int sched_waves_for = 0;  // wave count the cached schedule was built for
@@ -98,9 +105,10 @@ mf_status upload_tiles(mf_tiled *t, int device);
 void free_device(mf_tiled *t);
 // nnz-balanced contiguous tile-row partition for `nwaves` waves.
 std::vector<int> balance_tilerows(const mf_tiled *t, int nwaves);
-// Greedy split into waves of about target_nnz entries that each fit one SpMV
-// window (<= kWinRows tile-rows, <= kWinTiles tiles) unless a single tile-row
-// is larger than a window (such a tile-row gets a wave of its own).
+// Greedy split into waves of about target_nnz payload slots that each fit
+// one SpMV window (<= kWinRows tile-rows, <= kWinTiles tiles, <= kWinChunks
+// chunks) unless a single tile-row is larger than a window (such a tile-row
+// gets a wave of its own).  Needs pay_off.
 std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz);
 // Default stand-alone SpMV kernel (MF_SPMV_STREAM or MF_SPMV_ROWGROUP).
 int default_spmv_kernel(const mf_tiled *t);

@@ -294,7 +294,8 @@ __global__ void k_rows_av2(TileView A, const double *__restrict__ v, int nrows,
   double acc = 0.0;
   for (int t = A.tile_ptr[R]; t < A.tile_ptr[R + 1]; ++t) {
     const int xo = A.tile_col[t] << 4;
-    for (int k = A.tile_off[t]; k < A.tile_off[t + 1]; ++k) {
+    const int k0 = A.tile_off[t], k1 = k0 + A.tile_nz[t] + 1;
+    for (int k = k0; k < k1; ++k) {
       const int rc = A.rc[k];
       if ((rc >> 4) == r) acc += A.v64[k] * v[xo + (rc & 15)];
     }
@@ -311,7 +312,8 @@ __global__ void k_labels(TileView A, const double *__restrict__ v, double thr,
   double e32 = 0.0, e16 = 0.0, e8 = 0.0;  // sums of squared row errors
   double r32 = 0.0, r16 = 0.0, r8 = 0.0;  // running row errors
   int row = -1;
-  for (int k = A.tile_off[t]; k < A.tile_off[t + 1]; ++k) {
+  const int k0 = A.tile_off[t], k1 = k0 + A.tile_nz[t] + 1;
+  for (int k = k0; k < k1; ++k) {
     const int rc = A.rc[k];
     if ((rc >> 4) != row) {
       e32 += r32 * r32;

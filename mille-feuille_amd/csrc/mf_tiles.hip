@@ -2,13 +2,18 @@
 // (FP32 / FP16 / INT8 copies with per-tile scales, computed on the GPU) and
 // per-tile precision control.
 //
-// HBM layout per handle (T = tilenum, Z = nnz):
+// HBM layout per handle (T = tilenum, Z = nnz, ZP = payload slots: each
+// tile's entries padded to a multiple of 4, so that a 4-entry chunk never
+// straddles two tiles; Z <= ZP < Z + 3T):
 //   tile_ptr  (tilem+1) i32    tile-row -> first tile
 //   tile_col  T i32            tile column
-//   tile_off  (T+1) i32        payload offset of each tile (Z at the end)
+//   tile_off  (T+1) i32        payload offset of each tile, a multiple of 4
+//                              (ZP at the end)
 //   tile_row  T i32            tile-row of each tile
-//   rc        Z u8             (row << 4) | col of the entry inside its tile
-//   v64/v32/v16/v8             Z entries each: FP64, FP32, FP16 (scaled by
+//   tile_nz   T u8             nnz of the tile - 1
+//   rc        ZP u8            (row << 4) | col of the entry inside its tile
+//                              (padding repeats the tile's last byte)
+//   v64/v32/v16/v8             ZP entries each (padding 0): FP64, FP32, FP16 (scaled by
 //                              a power of two per tile), INT8 (scaled by a
 //                              power of two per tile)
 //   s16, s8   T f32            per-tile scales
@@ -32,7 +37,7 @@ namespace mf {
 // Payload arrays carry kPad extra zero entries: the streamed SpMV reads
 // entries in aligned groups of four and may touch up to three past the end.
 constexpr int kPad = 64;
-constexpr int kSpmvWaveNnz = 2048;
+constexpr int kSpmvWaveNnz = 1024;
 
 __global__ void k_pack_payloads(int T, const int *__restrict__ off, const double *__restrict__ v64,
                                 float *__restrict__ v32, uint16_t *__restrict__ v16,
@@ -99,15 +104,30 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   DeviceLayout &d = t->dev;
   MF_HIP(hipSetDevice(device));
   d.device = device;
-  const int T = t->tilenum, Z = t->csrsize, TM = t->tilem;
+  const int T = t->tilenum, TM = t->tilem;
   d.npad = kTile * std::max(t->tilem, t->tilen);
   if (d.npad == 0) d.npad = kTile;
 
-  // Per entry: (row << 4) | col inside the tile, from the tile's row starts.
-  // Per tile: the 16 row starts (rows past the last row of a short final
-  // tile-row start at the tile's nnz, which is then <= 15*16).
+  // Payload layout: each tile's entries start at a multiple of 4 (padded per
+  // tile to a multiple of 4 with zero values that repeat the tile's last
+  // (row,col) byte), so every 4-entry chunk of the streamed SpMV lies in one
+  // tile.  Per entry: (row << 4) | col inside the tile, from the tile's row
+  // starts.  Per tile: the 16 row starts (rows past the last row of a short
+  // final tile-row start at the tile's nnz, which is then <= 15*16) and
+  // nnz - 1.
+  std::vector<int> &poff = t->pay_off;
+  poff.assign(T + 1, 0);
+  for (int tt = 0; tt < T; ++tt) {
+    const int nzt = t->csr_offset[tt + 1] - t->csr_offset[tt];
+    poff[tt + 1] = poff[tt] + ((nzt + 3) & ~3);
+  }
+  const int ZP = poff[T];
+  d.npay = ZP;
   std::vector<int> tile_row(T);
-  std::vector<uint8_t> rc((size_t)Z + kPad, 0);
+  std::vector<uint8_t> rc((size_t)ZP + kPad, 0);
+  std::vector<uint8_t> tnz(T > 0 ? T : 1, 0);
+  std::vector<double> v64((size_t)ZP + kPad, 0.0);
+  std::vector<float> v32((size_t)ZP + kPad, 0.0f);
   std::vector<uint8_t> rp16((size_t)T * kTile + kPad, 0);
   std::vector<int> diag(TM, -1);
   int max_tiles = 0;
@@ -118,13 +138,20 @@ mf_status upload_tiles(mf_tiled *t, int device) {
       tile_row[tt] = bi;
       if (t->tile_columnidx[tt] == bi) diag[bi] = tt;
       const int off = t->csr_offset[tt], nzt = t->csr_offset[tt + 1] - off;
+      const int po = poff[tt];
+      tnz[tt] = (uint8_t)(nzt - 1);
       const unsigned char *rp = &t->Blockcsr_Ptr[t->csrptr_offset[tt]];
       for (int r = 0; r < kTile; ++r) rp16[(size_t)tt * kTile + r] = (uint8_t)(r < h ? rp[r] : nzt);
       for (int r = 0; r < h; ++r) {
         const int e = r == h - 1 ? nzt : rp[r + 1];
         for (int k = rp[r]; k < e; ++k)
-          rc[off + k] = (uint8_t)((r << 4) | t->Tile_csr_Col[off + k]);
+          rc[po + k] = (uint8_t)((r << 4) | t->Tile_csr_Col[off + k]);
       }
+      for (int k = 0; k < nzt; ++k) {
+        v64[po + k] = t->Blockcsr_Val[off + k];
+        v32[po + k] = t->Blockcsr_Val_Low[off + k];
+      }
+      for (int k = nzt; k < poff[tt + 1] - po; ++k) rc[po + k] = rc[po + nzt - 1];
     }
   }
   d.max_tiles_per_row = max_tiles;
@@ -138,12 +165,13 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_TRY(dalloc(&d.tile_col, T));
   MF_TRY(dalloc(&d.tile_off, T + 1));
   MF_TRY(dalloc(&d.tile_row, T));
-  MF_TRY(dalloc(&d.rc, (size_t)Z + kPad));
+  MF_TRY(dalloc(&d.tile_nz, T));
+  MF_TRY(dalloc(&d.rc, (size_t)ZP + kPad));
   MF_TRY(dalloc(&d.rp16, (size_t)T * kTile + kPad));
-  MF_TRY(dalloc(&d.v64, (size_t)Z + kPad));
-  MF_TRY(dalloc(&d.v32, (size_t)Z + kPad));
-  MF_TRY(dalloc(&d.v16, (size_t)Z + kPad));
-  MF_TRY(dalloc(&d.v8, (size_t)Z + kPad));
+  MF_TRY(dalloc(&d.v64, (size_t)ZP + kPad));
+  MF_TRY(dalloc(&d.v32, (size_t)ZP + kPad));
+  MF_TRY(dalloc(&d.v16, (size_t)ZP + kPad));
+  MF_TRY(dalloc(&d.v8, (size_t)ZP + kPad));
   if (nw > 0) MF_TRY(dalloc(&d.spmv_desc, nw));
   d.spmv_nwaves = nw;
   MF_TRY(dalloc(&d.s16, T));
@@ -155,15 +183,14 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_TRY(dalloc(&d.diag_tile, TM));
   MF_HIP(hipMemcpy(d.tile_ptr, t->tile_ptr.data(), sizeof(int) * (TM + 1), hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.tile_col, t->tile_columnidx.data(), sizeof(int) * T, hipMemcpyHostToDevice));
-  MF_HIP(hipMemcpy(d.tile_off, t->csr_offset.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.tile_off, poff.data(), sizeof(int) * (T + 1), hipMemcpyHostToDevice));
+  if (T > 0) MF_HIP(hipMemcpy(d.tile_nz, tnz.data(), T, hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.tile_row, tile_row.data(), sizeof(int) * T, hipMemcpyHostToDevice));
-  MF_HIP(hipMemcpy(d.rc, rc.data(), (size_t)Z + kPad, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.rc, rc.data(), (size_t)ZP + kPad, hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.rp16, rp16.data(), rp16.size(), hipMemcpyHostToDevice));
-  MF_HIP(hipMemset(d.v64, 0, sizeof(double) * ((size_t)Z + kPad)));
-  MF_HIP(hipMemset(d.v32, 0, sizeof(float) * ((size_t)Z + kPad)));
-  MF_HIP(hipMemset(d.v16, 0, sizeof(uint16_t) * ((size_t)Z + kPad)));
-  MF_HIP(hipMemset(d.v8, 0, (size_t)Z + kPad));
-  MF_HIP(hipMemcpy(d.v64, t->Blockcsr_Val.data(), sizeof(double) * Z, hipMemcpyHostToDevice));
+  MF_HIP(hipMemset(d.v16, 0, sizeof(uint16_t) * ((size_t)ZP + kPad)));
+  MF_HIP(hipMemset(d.v8, 0, (size_t)ZP + kPad));
+  MF_HIP(hipMemcpy(d.v64, v64.data(), sizeof(double) * ((size_t)ZP + kPad), hipMemcpyHostToDevice));
   if (nw > 0)
     MF_HIP(hipMemcpy(d.spmv_desc, wdesc.data(), sizeof(WaveDesc) * nw, hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.diag_tile, diag.data(), sizeof(int) * TM, hipMemcpyHostToDevice));
@@ -176,7 +203,7 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   }
   // The FP32 payload keeps the caller's FP32 copy when one was given
   // (the reference passes Val_Low, src/csr2block.h:248).
-  MF_HIP(hipMemcpy(d.v32, t->Blockcsr_Val_Low.data(), sizeof(float) * Z, hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(d.v32, v32.data(), sizeof(float) * ((size_t)ZP + kPad), hipMemcpyHostToDevice));
   MF_HIP(hipDeviceSynchronize());
   return MF_OK;
 }
@@ -197,7 +224,7 @@ int default_spmv_kernel(const mf_tiled *) {
 void free_device(mf_tiled *t) {
   DeviceLayout &d = t->dev;
   if (d.device >= 0) (void)hipSetDevice(d.device);
-  void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.rp16, d.spmv_desc,
+  void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.rp16, d.spmv_desc, d.tile_nz,
                   d.v64,      d.v32,      d.v16,      d.v8,       d.s16,    d.s8,
                   d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp, d.feat16,
                   t->sched.wave_tr, t->work};
