@@ -136,12 +136,15 @@ __device__ __forceinline__ double entry_product(const TileView &A, int lvl, int 
 // entry are padding.  The payload is loaded at the tile's level (FP64: 8
 // words, FP32: 4, FP16: 2, INT8: 1, packed); x is gathered once the (row,col)
 // bytes are in.
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
 struct Chunk {
   uint32_t rc4;
   int vmask;      // bit u: slot u holds an entry of the window; 0: none
   int j;          // window-relative tile
   int lvl;        // level of tile j
-  uint4 a, b;     // raw payload words 0-3 and 4-7
+  int wsel;       // fixed-width loads: FP16 / INT8 chunk's word in a (see chunk_load_pay_fixed)
+  u32x4_t a, b;   // raw payload words 0-3 and 4-7
   double x0, x1, x2, x3;  // x (FP64 tiles) or x32 (other levels) at each entry's column
 };
 
@@ -187,7 +190,7 @@ __device__ __forceinline__ void chunk_load_rc(const TileView &A, int Z0, int c, 
   const int k0 = Z0 + 4 * c;
   ch.rc4 = *(const uint32_t *)(A.rc + k0);
   if (all64) {
-    const uint4 *q = (const uint4 *)(A.v64 + k0);
+    const u32x4_t *q = (const u32x4_t *)(A.v64 + k0);
     ch.a = q[0];
     ch.b = q[1];
   }
@@ -199,7 +202,7 @@ __device__ __forceinline__ void chunk_load64_fixed(const TileView &A, int Z0, in
                                                    Chunk &ch) {
   const int k0 = Z0 + 4 * (c < nch ? c : nch - 1);
   ch.rc4 = *(const uint32_t *)(A.rc + k0);
-  const uint4 *q = (const uint4 *)(A.v64 + k0);
+  const u32x4_t *q = (const u32x4_t *)(A.v64 + k0);
   ch.a = q[0];
   ch.b = q[1];
 }
@@ -211,6 +214,92 @@ __device__ __forceinline__ void chunk_gather64_fixed(const WaveLds &L, const dou
   const int xo = L.tcol[ch.j];
 #pragma unroll
   for (int u = 0; u < 4; ++u) chunk_set_x(ch, u, x[xo + ((ch.rc4 >> (8 * u)) & 15)]);
+}
+
+// Stage 1c without a branch (after chunk_locate): two 16-byte loads at the
+// tile's level from 16-byte aligned addresses; FP64: the chunk's 32 bytes;
+// FP32: its 16 bytes (loaded twice); FP16: the 16 bytes holding its 8 bytes
+// (words 2*wsel, 2*wsel+1); INT8 / SKIP: the 16 bytes holding its 4 bytes
+// (word wsel).  A chunk past the window loads the window's last chunk.
+__device__ __forceinline__ void chunk_load_pay_fixed(const TileView &A, int Z0, int c, int nch,
+                                                     Chunk &ch) {
+  const int k0 = Z0 + 4 * (c < nch ? c : nch - 1);
+  const int lv = ch.lvl;
+  const char *p64 = (const char *)(A.v64 + k0);
+  const char *p32 = (const char *)(A.v32 + k0);
+  const char *p16 = (const char *)(A.v16 + (k0 & ~7));
+  const char *p8 = (const char *)(A.v8 + (k0 & ~15));
+  const char *p = lv == MF_PREC_FP64 ? p64 : lv == MF_PREC_FP32 ? p32 : lv == MF_PREC_FP16 ? p16 : p8;
+  ch.wsel = lv == MF_PREC_FP16 ? (k0 >> 2) & 1 : (k0 >> 2) & 3;
+  ch.a = *(const u32x4_t *)p;
+  ch.b = *(const u32x4_t *)(p + (lv == MF_PREC_FP64 ? 16 : 0));
+}
+
+__device__ __forceinline__ void chunk_load_lv_fixed(const TileView &A, int Z0, int c, int nch,
+                                                    Chunk &ch) {
+  ch.rc4 = *(const uint32_t *)(A.rc + Z0 + 4 * (c < nch ? c : nch - 1));
+  chunk_load_pay_fixed(A, Z0, c, nch, ch);
+}
+
+// Stage 2 without a branch, per-tile levels: for every slot an 8-byte load,
+// x[xi] for FP64 tiles, else the aligned pair of x32 holding x32[xi] (the
+// half is picked by the column's low bit when consumed).
+__device__ __forceinline__ void chunk_gather_lv_fixed(const WaveLds &L,
+                                                      const double *__restrict__ x,
+                                                      const float *__restrict__ x32, Chunk &ch) {
+  const int xo = L.tcol[ch.j];
+  const bool wide = ch.lvl == MF_PREC_FP64;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int xi = xo + ((ch.rc4 >> (8 * u)) & 15);
+    const uint2 *q = wide ? (const uint2 *)(x + xi) : (const uint2 *)(x32 + (xi & ~1));
+    const uint2 v = *q;
+    chunk_set_x(ch, u, __builtin_bit_cast(double, ((uint64_t)v.y << 32) | v.x));
+  }
+}
+
+// Stage 3 for the fixed-width loads (per-tile levels).
+__device__ __forceinline__ void chunk_consume_lv_fixed(WaveLds &L, const Chunk &ch) {
+  if (!ch.vmask || ch.lvl == MF_PREC_SKIP) return;
+  const int ro = L.trow[ch.j] * 16;
+  const int lv = ch.lvl;
+  const float sc = lv == MF_PREC_FP16 || lv == MF_PREC_INT8 ? L.tscale[ch.j] : 1.0f;
+  const uint32_t w0 = ch.wsel == 0 ? ch.a.x : ch.wsel == 1 ? ch.a.y : ch.wsel == 2 ? ch.a.z : ch.a.w;
+  const uint32_t h0 = ch.wsel == 0 ? ch.a.x : ch.a.z;  // FP16: words 2*wsel, 2*wsel+1
+  const uint32_t h1 = ch.wsel == 0 ? ch.a.y : ch.a.w;
+  int key = -1;
+  double run = 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (!(ch.vmask >> u & 1)) continue;
+    const int col = (ch.rc4 >> (8 * u)) & 15;
+    const int kk = ro + ((ch.rc4 >> (8 * u + 4)) & 15);
+    const uint64_t raw = __builtin_bit_cast(uint64_t, chunk_x(ch, u));
+    double prod;
+    if (lv == MF_PREC_FP64) {
+      prod = __builtin_bit_cast(double, ((uint64_t)chunk_word(ch, 2 * u + 1) << 32) |
+                                            chunk_word(ch, 2 * u)) * __builtin_bit_cast(double, raw);
+    } else {
+      const float xf = __builtin_bit_cast(float, (uint32_t)(col & 1 ? raw >> 32 : raw));
+      if (lv == MF_PREC_FP32) {
+        prod = (double)(__builtin_bit_cast(float, chunk_word(ch, u)) * xf);
+      } else if (lv == MF_PREC_FP16) {
+        const uint16_t h = (uint16_t)((u < 2 ? h0 : h1) >> (16 * (u & 1)));
+        prod = (double)(half_bits_to_float(h) * xf * sc);
+      } else {
+        const int8_t q = (int8_t)(w0 >> (8 * u));
+        prod = (double)((float)q * xf * sc);
+      }
+    }
+    if (kk != key) {
+      if (key >= 0) atomicAdd(&L.acc[key], run);
+      key = kk;
+      run = prod;
+    } else {
+      run += prod;
+    }
+  }
+  atomicAdd(&L.acc[key], run);
 }
 
 // Stage 1b: tile, level and valid slots of the chunk (LDS only).
@@ -233,11 +322,11 @@ __device__ __forceinline__ void chunk_load_payload(const TileView &A, int Z0, in
   const int k0 = Z0 + 4 * c;
   const int lv = ch.lvl;
   if (lv == MF_PREC_FP64) {
-    const uint4 *q = (const uint4 *)(A.v64 + k0);
+    const u32x4_t *q = (const u32x4_t *)(A.v64 + k0);
     ch.a = q[0];
     ch.b = q[1];
   } else if (lv == MF_PREC_FP32) {
-    ch.a = *(const uint4 *)(A.v32 + k0);
+    ch.a = *(const u32x4_t *)(A.v32 + k0);
   } else if (lv == MF_PREC_FP16) {
     const uint2 h = *(const uint2 *)(A.v16 + k0);
     ch.a.x = h.x;
@@ -447,6 +536,39 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
           chunk_gather64_fixed(L, x, cz);
           if (m + 4 < nsl) chunk_load64_fixed(A, Z0, c0 + 256, nch, cy);
           chunk_consume(L, cz, true);
+        }
+      } else if (DEPTH == 3 && MODE != kAll64) {
+        // Three-deep, per-tile levels, fixed-width loads: slice m+2 is
+        // located (LDS) and its loads issued after slice m's gathers.
+        Chunk cz;
+        chunk_locate(L, lane, nch, all64, cx);
+        chunk_load_pay_fixed(A, Z0, lane, nch, cx);
+        if (nsl > 1) {
+          chunk_locate(L, lane + 64, nch, all64, cy);
+          chunk_load_lv_fixed(A, Z0, lane + 64, nch, cy);
+        }
+        for (int m = 0; m < nsl; m += 3) {
+          const int c0 = 64 * m + lane;
+          chunk_gather_lv_fixed(L, x, x32, cx);
+          if (m + 2 < nsl) {
+            chunk_locate(L, c0 + 128, nch, all64, cz);
+            chunk_load_lv_fixed(A, Z0, c0 + 128, nch, cz);
+          }
+          chunk_consume_lv_fixed(L, cx);
+          if (m + 1 >= nsl) break;
+          chunk_gather_lv_fixed(L, x, x32, cy);
+          if (m + 3 < nsl) {
+            chunk_locate(L, c0 + 192, nch, all64, cx);
+            chunk_load_lv_fixed(A, Z0, c0 + 192, nch, cx);
+          }
+          chunk_consume_lv_fixed(L, cy);
+          if (m + 2 >= nsl) break;
+          chunk_gather_lv_fixed(L, x, x32, cz);
+          if (m + 4 < nsl) {
+            chunk_locate(L, c0 + 256, nch, all64, cy);
+            chunk_load_lv_fixed(A, Z0, c0 + 256, nch, cy);
+          }
+          chunk_consume_lv_fixed(L, cz);
         }
       } else if (all64) {
         for (int m = 0; m < nsl; m += 2) {
