@@ -219,6 +219,9 @@ typedef struct mf_solve_opts {
                            in HBM, three syncs per iteration) or MF_FORM_RESIDENT
                            (vectors in registers/LDS, two syncs; needs regular
                            tile-rows and at most 8192 rows per CU) */
+  int batch_waves;      /* mf_batch_cg_solve: waves per (system, right-hand
+                           side) item, 4, 8 or 16 (0 -> 8; mf_tune_batch
+                           picks one) */
   void *stream;         /* hipStream_t */
 } mf_solve_opts;
 
@@ -376,6 +379,39 @@ mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs,
                             const mf_solve_opts *opts, int *iterations,
                             double *rel_residual, double *per_system_ms,
                             double *total_ms);
+
+/* ---------------------------------------------------------------------- */
+/* Autotuner                                                               */
+/* ---------------------------------------------------------------------- */
+
+/* Candidate schedules timed on the device; the fastest is kept.  cand[i] is
+ * a candidate's value and us[i] its median time in microseconds (SpMV: one
+ * launch; batch: the whole timed batch run). */
+typedef struct mf_tune_report {
+  int ncand;
+  int cand[8];
+  float us[8];
+  int best;             /* chosen value */
+} mf_tune_report;
+
+/* Stand-alone SpMV schedule of a handle for mf_spmv `mode` (0 or 1, the
+ * handle's current levels): the streamed kernel at 384, 512, 768, 1024, 1536
+ * and 2048 entries per wave and, on matrices with regular tile-rows, the
+ * row-group kernel (value 0).  The fastest is applied with
+ * mf_tile_set_schedule / mf_tile_set_spmv_kernel.  Allocates and frees its
+ * own x, x32 and y.
+ * No reference counterpart (the reference's warp budget is fixed,
+ * src/main-cg.cu:634-675). */
+mf_status mf_tune_spmv(mf_tiled *t, int mode, mf_tune_report *rep);
+
+/* Waves per (system, right-hand side) item of mf_batch_cg_solve: 4, 8 and
+ * 16 are timed on `iters` CG iterations of the given batch (on copies of the
+ * initial guesses d_x, which are left unchanged); rep->best is the value for
+ * opts->batch_waves.
+ * No reference counterpart. */
+mf_status mf_tune_batch(mf_tiled **systems, int nsys, int nrhs, const double **d_b,
+                        double **d_x, const mf_solve_opts *opts, int iters,
+                        mf_tune_report *rep);
 
 const char *mf_last_error(void);
 const char *mf_version(void);

@@ -29,12 +29,13 @@
 
 namespace mf {
 
-constexpr int kBatchBlock = 512;
-constexpr int kBatchWaves = kBatchBlock / 64;
+// Waves per (system, right-hand side) item: 4, 8 (default) or 16, picked by
+// opts->batch_waves or by mf_tune_batch.
+constexpr int kBatchWavesDefault = 8;
 
 struct BatchSys {
   TileView A;
-  const int *wave_tr;  // kBatchWaves + 1 tile-row bounds
+  const int *wave_tr;  // waves per item + 1 tile-row bounds
   const double *b;     // npad * nrhs, right-hand side j at j * npad
   double *x;           // npad * nrhs
   int npad;
@@ -69,10 +70,11 @@ __global__ void k_batch_reset(int *head, int nitems, int *iters, double *relres)
   }
 }
 
-__global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
-  __shared__ double lds[kBatchWaves];
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch_cg(BatchArgs a) {
+  __shared__ double lds[BLOCK / 64];
   __shared__ int item_s;
-  __shared__ WaveLds wl[kBatchWaves];
+  __shared__ WaveLds wl[BLOCK / 64];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   WaveLds &L = wl[wv];
@@ -99,13 +101,13 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
     const int tr1 = __builtin_amdgcn_readfirstlane(S.wave_tr[wv + 1]);
     const int row0 = tr0 * 16, row1 = tr1 * 16;
     // gathered padding of p beyond the tile-rows must be zero
-    for (int i = 16 * A.tilem + threadIdx.x; i < npad; i += kBatchBlock) {
+    for (int i = 16 * A.tilem + threadIdx.x; i < npad; i += BLOCK) {
       p[i] = 0.0;
       p32[i] = 0.0f;
     }
     // r = b - A x, p = r
     double rr0 = 0.0, bb0 = 0.0;
-    wave_spmv<kAll64, false, 2>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+    wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
       [&](const int i, const double ax) {
         const double bv = b[i], rv = bv - ax;
         r[i] = rv;
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
     while (!done) {
       if (restart) {
         double tr0v = 0.0;
-        wave_spmv<kAll64, false, 2>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+        wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
           [&](const int i, const double ax) {
             const double rv = b[i] - ax;
             r[i] = rv;
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
       }
       ++it;
       double pq0 = 0.0;
-      wave_spmv<kPerTile, false, 2>(A, tr0, tr1, p, p32, nullptr, force64, L, lane, &cnt,
+      wave_spmv<kPerTile, false>(A, tr0, tr1, p, p32, nullptr, force64, L, lane, &cnt,
         [&](const int i, const double qv) {
           q[i] = qv;
           pq0 += p[i] * qv;
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
                             (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
       if (converged || periodic) {
         double tv = 0.0;
-        wave_spmv<kAll64, false, 2>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+        wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
           [&](const int i, const double ax) {
             const double rv = b[i] - ax;
             r[i] = rv;
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_cg(BatchArgs a) {
     __syncthreads();
     // FP64 relative residual of the result
     double fv = 0.0;
-    wave_spmv<kAll64, false, 2>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
+    wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
       [&](const int i, const double ax) {
         const double d = b[i] - ax;
         fv += d * d;
@@ -281,15 +283,19 @@ extern "C" mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs, c
 
   // per-system levels, schedules and descriptors
   std::vector<BatchSys> hs(nsys);
-  std::vector<int> htr((size_t)nsys * (kBatchWaves + 1));
+  const int nwv = opt.batch_waves == 0 ? kBatchWavesDefault : opt.batch_waves;
+  if (nwv != 4 && nwv != 8 && nwv != 16)
+    return fail(MF_ERR_ARG, "mf_batch_cg_solve: batch_waves must be 0, 4, 8 or 16");
+  const int block = 64 * nwv;
+  std::vector<int> htr((size_t)nsys * (nwv + 1));
   int npad_max = 16;
   for (int i = 0; i < nsys; ++i) {
     mf_tiled *t = systems[i];
     if (opt.policy == MF_POLICY_FIXED)
       MF_HIP(hipMemcpyAsync(t->dev.prec, t->dev.prec_user, t->tilenum, hipMemcpyDeviceToDevice, st));
     if (opt.policy == MF_POLICY_ERRBOUND) MF_TRY(apply_errbound_levels(t, st));
-    const std::vector<int> w = balance_tilerows(t, kBatchWaves);
-    std::copy(w.begin(), w.end(), htr.begin() + (size_t)i * (kBatchWaves + 1));
+    const std::vector<int> w = balance_tilerows(t, nwv);
+    std::copy(w.begin(), w.end(), htr.begin() + (size_t)i * (nwv + 1));
     npad_max = std::max(npad_max, t->dev.npad);
   }
   int *d_tr = nullptr;
@@ -297,7 +303,7 @@ extern "C" mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs, c
   MF_HIP(hipMemcpyAsync(d_tr, htr.data(), sizeof(int) * htr.size(), hipMemcpyHostToDevice, st));
   for (int i = 0; i < nsys; ++i) {
     hs[i].A = view_of(systems[i]);
-    hs[i].wave_tr = d_tr + (size_t)i * (kBatchWaves + 1);
+    hs[i].wave_tr = d_tr + (size_t)i * (nwv + 1);
     hs[i].b = d_b[i];
     hs[i].x = d_x[i];
     hs[i].npad = systems[i]->dev.npad;
@@ -316,7 +322,9 @@ extern "C" mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs, c
 
   int ncu = 0, nb = 0;
   MF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_cg, kBatchBlock, 0));
+  const void *kern = nwv == 4 ? (const void *)k_batch_cg<256>
+                   : nwv == 8 ? (const void *)k_batch_cg<512> : (const void *)k_batch_cg<1024>;
+  MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, block, 0));
   if (nb < 1) nb = 1;
   int G = std::min(nitems, ncu * nb);
   if (opt.blocks > 0) G = std::min(G, opt.blocks);
@@ -358,7 +366,9 @@ extern "C" mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs, c
   // capture: queue reset (a kernel, not a memset node) + the persistent solve
   MF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   k_batch_reset<<<(nitems + 255) / 256, 256, 0, st>>>(d_head, nitems, d_iters, d_rel);
-  k_batch_cg<<<G, kBatchBlock, 0, st>>>(args);
+  if (nwv == 4) k_batch_cg<256><<<G, 256, 0, st>>>(args);
+  else if (nwv == 8) k_batch_cg<512><<<G, 512, 0, st>>>(args);
+  else k_batch_cg<1024><<<G, 1024, 0, st>>>(args);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(st, &g);
   own_res.g = g;

@@ -132,7 +132,7 @@ __device__ __forceinline__ void flush_hist(const PrecCount &c, unsigned long lon
 __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int tr0, int tr1,
                                                      int lane, double *store, WaveLds *wl) {
   double acc = 0.0;
-  wave_spmv<kAll64, false, 2>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
       const double rv = a.b[i] - ax;
       if (store) store[i] = rv;
@@ -146,7 +146,7 @@ __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int tr0
 __device__ __forceinline__ double restart_rows(const SolveArgs &a, int tr0, int tr1, int lane,
                                                bool mixed, bool bicg, WaveLds *wl) {
   double acc = 0.0;
-  wave_spmv<kAll64, false, 2>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
       const double rv = a.b[i] - ax;
       a.r[i] = rv;
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
 
   // r = b - A x0, p = r.
   double red[2] = {0.0, 0.0};
-  wave_spmv<kAll64, false, 2>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
       const double bv = a.b[i], rv = bv - ax;
       a.r[i] = rv;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
     double pq[1] = {0.0};
     if (infer && !force64) wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
     infer = false;
-    wave_spmv<MODE, mixed, 2>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
+    wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
       [&](const int i, const double qv) {
         a.q[i] = qv;
         pq[0] += a.p[i] * qv;
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   double *v = a.q, *t = a.tv;
 
   double red[2] = {0.0, 0.0};
-  wave_spmv<kAll64, false, 2>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
       const double bv = a.b[i], rv = bv - ax;
       a.r[i] = rv;
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     double vr[1] = {0.0};
     if (infer && !force64) wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
     infer = false;
-    wave_spmv<MODE, mixed, 2>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
+    wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
       [&](const int i, const double vv) {
         v[i] = vv;
         vr[0] += vv * a.rh[i];
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     grid_sync(a.gs, epoch, &lds_i);
     // t = A s, t.s and t.t
     double tw[2] = {0.0, 0.0};
-    wave_spmv<MODE, mixed, 2>(a.A, tr0, tr1, a.s, a.s32, a.seg_s, force64, wl[threadIdx.x >> 6], lane, &cnt,
+    wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.s, a.s32, a.seg_s, force64, wl[threadIdx.x >> 6], lane, &cnt,
       [&](const int i, const double tvv) {
         t[i] = tvv;
         tw[0] += tvv * a.s[i];
@@ -454,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
                             (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
       if (converged || periodic || omega == 0.0 || !isfinite(rr_new)) {
         double tr[2] = {0.0, 0.0};
-        wave_spmv<kAll64, false, 2>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+        wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
           [&](const int i, const double ax) {
             const double rv = a.b[i] - ax;
             a.r[i] = rv;

@@ -47,7 +47,7 @@ class MfSolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("policy", C.c_int),
                 ("check_interval", C.c_int), ("refresh_ratio", C.c_double),
                 ("max_refresh", C.c_int), ("refresh_every", C.c_int), ("blocks", C.c_int),
-                ("fixed_iters", C.c_int), ("form", C.c_int),
+                ("fixed_iters", C.c_int), ("form", C.c_int), ("batch_waves", C.c_int),
                 ("stream", C.c_void_p)]
 
 
@@ -66,6 +66,14 @@ class MfReport(C.Structure):
                 "time_ms": self.time_ms, "prec_hist": list(self.prec_hist),
                 "grid_blocks": self.grid_blocks, "form_used": self.form_used,
                 "policy_used": self.policy_used}
+
+
+class MfTuneReport(C.Structure):
+    _fields_ = [("ncand", C.c_int), ("cand", C.c_int * 8), ("us", C.c_float * 8), ("best", C.c_int)]
+
+    def as_dict(self):
+        n = self.ncand
+        return {"cand": list(self.cand[:n]), "us": [float(v) for v in self.us[:n]], "best": self.best}
 
 
 _vp = C.c_void_p
@@ -127,6 +135,9 @@ SIGNATURES = {
     "mf_policy_set_mlp": (C.c_int, [_vp, _vp]),
     "mf_policy_features": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp]),
     "mf_policy_labels": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp]),
+    "mf_tune_spmv": (C.c_int, [_vp, C.c_int, C.POINTER(MfTuneReport)]),
+    "mf_tune_batch": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(MfSolveOpts), C.c_int,
+                                C.POINTER(MfTuneReport)]),
     "mf_batch_cg_solve": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, C.POINTER(MfSolveOpts), _vp,
                                     _vp, _vp, C.POINTER(C.c_double)]),
     "mf_last_error": (C.c_char_p, []),

@@ -39,7 +39,7 @@ _FORMS = {"auto": _lib.FORM_AUTO, "memory": _lib.FORM_MEMORY, "resident": _lib.F
 
 
 def solve_opts(tol=1e-10, maxiter=10000, policy="fp64", check_interval=8, refresh_ratio=0.1,
-               max_refresh=64, refresh_every=64, blocks=0, fixed_iters=0, form="auto"):
+               max_refresh=64, refresh_every=64, blocks=0, fixed_iters=0, form="auto", batch_waves=0):
     o = MfSolveOpts()
     lib.mf_solve_opts_default(C.byref(o))
     o.tol = tol
@@ -52,6 +52,7 @@ def solve_opts(tol=1e-10, maxiter=10000, policy="fp64", check_interval=8, refres
     o.blocks = blocks
     o.fixed_iters = fixed_iters
     o.form = _FORMS[form] if isinstance(form, str) else int(form)
+    o.batch_waves = int(batch_waves)
     return o
 
 
@@ -150,6 +151,16 @@ class TiledMatrix:
     def set_schedule(self, spmv_wave_nnz):
         check("mf_tile_set_schedule", lib.mf_tile_set_schedule(self.h, int(spmv_wave_nnz)))
 
+    def tune_spmv(self, mode=0):
+        """Autotunes the stand-alone SpMV (mf_tune_spmv) for `mode` at the
+        handle's current levels and keeps the fastest schedule; returns the
+        candidates' timings."""
+        rep = _lib.MfTuneReport()
+        check("mf_tune_spmv", lib.mf_tune_spmv(self.h, int(mode), C.byref(rep)))
+        d = rep.as_dict()
+        self._spmv_kernel = _lib.SPMV_ROWGROUP if d["best"] == 0 else _lib.SPMV_STREAM
+        return d
+
     def spmv_kernel_name(self):
         """Device kernel that mf_spmv launches for this handle (mode 0)."""
         return "mf::k_spmv_rg<0>" if self._spmv_kernel == _lib.SPMV_ROWGROUP else "mf::k_spmv<0>"
@@ -231,12 +242,15 @@ def spmv_bytes(tm: "TiledMatrix", levels=None, x_bytes=8):
     return payload + 24 * active + 4 * scaled + n * x_bytes + 8 * n
 
 
-def batch_cg(systems, bs, nrhs=1, x0=None, allow_not_converged=False, **kw):
+def batch_cg(systems, bs, nrhs=1, x0=None, allow_not_converged=False, tune_iters=0, **kw):
     """Solves every system in `systems` (TiledMatrix handles on one device)
     for `nrhs` right-hand sides each, in one persistent launch captured in a
     hipGraph (mf_batch_cg_solve).  bs[i] is an (nrhs, rowA_i) or (rowA_i,)
-    array.  Returns (xs, report) with xs[i] of shape (nrhs, rowA_i) and a
-    report holding iterations, relative residuals, per-system and total ms."""
+    array.  tune_iters > 0 first picks the waves per item with mf_tune_batch
+    on that many iterations (unless batch_waves is given).  Returns (xs,
+    report) with xs[i] of shape (nrhs, rowA_i) and a report holding
+    iterations, relative residuals, per-system and total ms (and the tuner's
+    timings under "tune")."""
     opts = solve_opts(**kw)
     nsys = len(systems)
     db, dx = [], []
@@ -253,6 +267,13 @@ def batch_cg(systems, bs, nrhs=1, x0=None, allow_not_converged=False, **kw):
     hs = (C.c_void_p * nsys)(*[tm.h.value for tm in systems])
     pb = (C.c_void_p * nsys)(*[d.ptr for d in db])
     px = (C.c_void_p * nsys)(*[d.ptr for d in dx])
+    tune = None
+    if tune_iters > 0 and not kw.get("batch_waves"):
+        rep = _lib.MfTuneReport()
+        check("mf_tune_batch", lib.mf_tune_batch(hs, nsys, nrhs, pb, px, C.byref(opts), int(tune_iters),
+                                                 C.byref(rep)))
+        tune = rep.as_dict()
+        opts.batch_waves = tune["best"]
     its = np.zeros(nsys * nrhs, np.int32)
     rel = np.zeros(nsys * nrhs, np.float64)
     per = np.zeros(nsys, np.float64)
@@ -263,4 +284,4 @@ def batch_cg(systems, bs, nrhs=1, x0=None, allow_not_converged=False, **kw):
     xs = [d.get().reshape(nrhs, tm.npad)[:, : tm.rowA] for d, tm in zip(dx, systems)]
     return xs, {"status": code, "iterations": its.reshape(nsys, nrhs),
                 "rel_residual": rel.reshape(nsys, nrhs), "per_system_ms": per,
-                "total_ms": tot.value}
+                "total_ms": tot.value, "batch_waves": opts.batch_waves or 8, "tune": tune}
