@@ -98,7 +98,9 @@ struct PrecCount {
 // Mixed precision: FP64 tiles multiply x; FP32 / FP16 / INT8 tiles multiply
 // the FP32 copy x32 with FP32 products (FP16/INT8 scaled by the tile's power
 // of two) that are accumulated in FP64; SKIP tiles contribute nothing.
-// force64 != 0 treats every tile as FP64 (the solver's fallback).
+// force64 bit 0 treats every tile as FP64 (the solver's fallback); bit 1
+// (per-tile levels) makes FP64 tiles multiply the FP32 copy x32 as well, in
+// FP64 arithmetic (the mixed solvers keep their vectors in FP32 only).
 // ---------------------------------------------------------------------------
 
 struct WaveLds {
@@ -114,7 +116,7 @@ struct WaveLds {
 template <int MODE>
 __device__ __forceinline__ int tile_level(const TileView &A, const uint8_t *seglev, int t,
                                           int force64) {
-  if (MODE == kAll64 || force64) return MF_PREC_FP64;
+  if (MODE == kAll64 || (force64 & 1)) return MF_PREC_FP64;
   if (MODE == kPerTile) return A.prec[t];
   return seglev[A.tile_col[t]];
 }
@@ -246,9 +248,10 @@ __device__ __forceinline__ void chunk_load_lv_fixed(const TileView &A, int Z0, i
 // half is picked by the column's low bit when consumed).
 __device__ __forceinline__ void chunk_gather_lv_fixed(const WaveLds &L,
                                                       const double *__restrict__ x,
-                                                      const float *__restrict__ x32, Chunk &ch) {
+                                                      const float *__restrict__ x32, Chunk &ch,
+                                                      bool x32only) {
   const int xo = L.tcol[ch.j];
-  const bool wide = ch.lvl == MF_PREC_FP64;
+  const bool wide = ch.lvl == MF_PREC_FP64 && !x32only;
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int xi = xo + ((ch.rc4 >> (8 * u)) & 15);
@@ -259,7 +262,7 @@ __device__ __forceinline__ void chunk_gather_lv_fixed(const WaveLds &L,
 }
 
 // Stage 3 for the fixed-width loads (per-tile levels).
-__device__ __forceinline__ void chunk_consume_lv_fixed(WaveLds &L, const Chunk &ch) {
+__device__ __forceinline__ void chunk_consume_lv_fixed(WaveLds &L, const Chunk &ch, bool x32only) {
   if (!ch.vmask || ch.lvl == MF_PREC_SKIP) return;
   const int ro = L.trow[ch.j] * 16;
   const int lv = ch.lvl;
@@ -275,12 +278,13 @@ __device__ __forceinline__ void chunk_consume_lv_fixed(WaveLds &L, const Chunk &
     const int col = (ch.rc4 >> (8 * u)) & 15;
     const int kk = ro + ((ch.rc4 >> (8 * u + 4)) & 15);
     const uint64_t raw = __builtin_bit_cast(uint64_t, chunk_x(ch, u));
+    const float xf = __builtin_bit_cast(float, (uint32_t)(col & 1 ? raw >> 32 : raw));
     double prod;
     if (lv == MF_PREC_FP64) {
       prod = __builtin_bit_cast(double, ((uint64_t)chunk_word(ch, 2 * u + 1) << 32) |
-                                            chunk_word(ch, 2 * u)) * __builtin_bit_cast(double, raw);
+                                            chunk_word(ch, 2 * u)) *
+             (x32only ? (double)xf : __builtin_bit_cast(double, raw));
     } else {
-      const float xf = __builtin_bit_cast(float, (uint32_t)(col & 1 ? raw >> 32 : raw));
       if (lv == MF_PREC_FP32) {
         prod = (double)(__builtin_bit_cast(float, chunk_word(ch, u)) * xf);
       } else if (lv == MF_PREC_FP16) {
@@ -413,7 +417,8 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
   const int tr1 = __builtin_amdgcn_readfirstlane(desc.tr1);
   if (tr0 >= tr1) return;
   const int T1 = __builtin_amdgcn_readfirstlane(desc.T1);
-  const bool all64 = MODE == kAll64 || force64;
+  const bool all64 = MODE == kAll64 || (force64 & 1);
+  const bool x32only = MODE != kAll64 && !all64 && (force64 & 2);
   for (int i = lane; i < kWinRows * 16; i += 64) L.acc[i] = 0.0;
   int ta = __builtin_amdgcn_readfirstlane(desc.ta);
   int Za = __builtin_amdgcn_readfirstlane(desc.Z0);
@@ -549,26 +554,26 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
         }
         for (int m = 0; m < nsl; m += 3) {
           const int c0 = 64 * m + lane;
-          chunk_gather_lv_fixed(L, x, x32, cx);
+          chunk_gather_lv_fixed(L, x, x32, cx, x32only);
           if (m + 2 < nsl) {
             chunk_locate(L, c0 + 128, nch, all64, cz);
             chunk_load_lv_fixed(A, Z0, c0 + 128, nch, cz);
           }
-          chunk_consume_lv_fixed(L, cx);
+          chunk_consume_lv_fixed(L, cx, x32only);
           if (m + 1 >= nsl) break;
-          chunk_gather_lv_fixed(L, x, x32, cy);
+          chunk_gather_lv_fixed(L, x, x32, cy, x32only);
           if (m + 3 < nsl) {
             chunk_locate(L, c0 + 192, nch, all64, cx);
             chunk_load_lv_fixed(A, Z0, c0 + 192, nch, cx);
           }
-          chunk_consume_lv_fixed(L, cy);
+          chunk_consume_lv_fixed(L, cy, x32only);
           if (m + 2 >= nsl) break;
-          chunk_gather_lv_fixed(L, x, x32, cz);
+          chunk_gather_lv_fixed(L, x, x32, cz, x32only);
           if (m + 4 < nsl) {
             chunk_locate(L, c0 + 256, nch, all64, cy);
             chunk_load_lv_fixed(A, Z0, c0 + 256, nch, cy);
           }
-          chunk_consume_lv_fixed(L, cz);
+          chunk_consume_lv_fixed(L, cz, x32only);
         }
       } else if (all64) {
         for (int m = 0; m < nsl; m += 2) {

@@ -51,6 +51,7 @@ struct SolveArgs {
   double *r, *p, *q;          // CG (q doubles as v in BiCGSTAB)
   double *rh, *s, *tv;        // BiCGSTAB
   float *p32, *s32;
+  float *r32, *q32;           // mixed CG: r and q in FP32 (p in p32)
   uint8_t *seg_p, *seg_s;     // per-segment levels (rule policy)
   const int *wave_tr;
   double *partials;
@@ -127,15 +128,22 @@ __device__ __forceinline__ void flush_hist(const PrecCount &c, unsigned long lon
   }
 }
 
-// True residual of the wave's tile-rows: r_new = b - A x (FP64), returned
-// per lane (lanes 0..15) and optionally stored.  Accumulates ||r_new||^2.
+// True residual of the wave's tile-rows: r_new = b - A x (FP64), optionally
+// stored in FP64 (store) or FP32 (store32).  Returns the wave's share of
+// ||r_new||^2 (of the stored FP32 values when store32 is given).
 __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int tr0, int tr1,
-                                                     int lane, double *store, WaveLds *wl) {
+                                                     int lane, double *store, WaveLds *wl,
+                                                     float *store32 = nullptr) {
   double acc = 0.0;
   wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
-      const double rv = a.b[i] - ax;
+      double rv = a.b[i] - ax;
       if (store) store[i] = rv;
+      if (store32) {
+        const float rf = (float)rv;
+        store32[i] = rf;
+        rv = rf;
+      }
       acc += rv * rv;
     });
   return acc;
@@ -143,16 +151,25 @@ __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int tr0
 
 // Restart helper: r = b - A x in FP64 for the wave's tile-rows, p = r (and
 // r^ = r for BiCGSTAB).  Returns the wave's share of ||r||^2.
+// lowv: the mixed CG's FP32 vectors r32 = p32 = (float) r instead.
 __device__ __forceinline__ double restart_rows(const SolveArgs &a, int tr0, int tr1, int lane,
-                                               bool mixed, bool bicg, WaveLds *wl) {
+                                               bool mixed, bool bicg, WaveLds *wl,
+                                               bool lowv = false) {
   double acc = 0.0;
   wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
-      const double rv = a.b[i] - ax;
-      a.r[i] = rv;
-      a.p[i] = rv;
-      if (mixed) a.p32[i] = (float)rv;
-      if (bicg) a.rh[i] = rv;
+      double rv = a.b[i] - ax;
+      if (lowv) {
+        const float rf = (float)rv;
+        a.r32[i] = rf;
+        a.p32[i] = rf;
+        rv = rf;
+      } else {
+        a.r[i] = rv;
+        a.p[i] = rv;
+        if (mixed) a.p32[i] = (float)rv;
+        if (bicg) a.rh[i] = rv;
+      }
       acc += rv * rv;
     });
   return acc;
@@ -173,14 +190,21 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   int slot = 0;
   PrecCount cnt;
 
-  // r = b - A x0, p = r.
+  // r = b - A x0, p = r (FP64, and the FP32 copies the mixed recurrence
+  // starts from).
   double red[2] = {0.0, 0.0};
   wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
-      const double bv = a.b[i], rv = bv - ax;
+      const double bv = a.b[i];
+      double rv = bv - ax;
       a.r[i] = rv;
       a.p[i] = rv;
-      if (mixed) a.p32[i] = (float)rv;
+      if (mixed) {
+        const float rf = (float)rv;
+        a.p32[i] = rf;
+        a.r32[i] = rf;
+        rv = rf;
+      }
       red[0] += rv * rv;
       red[1] += bv * bv;
     });
@@ -188,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
     // segment norms of p = r0 for the first inference
     for (int base = row0; base < row1; base += 64) {
       const int i = base + lane;
-      const double pv = i < row1 ? a.p[i] : 0.0;
+      const double pv = i < row1 ? (double)a.p32[i] : 0.0;
       const double d2 = group16_sum(pv * pv);
       if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d2;
     }
@@ -204,10 +228,14 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   bool infer = learned;  // learned policy: new levels in the next SpMV
   double pp_last = rr;   // ||p||^2 at the last inference point
 
+  // Mixed policies keep r, p and q in FP32 (x, the scalars and the
+  // residual recomputations stay FP64); after a fallback to FP64 tiles the
+  // recurrence restarts in FP64 vectors.
   while (!done) {
+    const bool lowv = mixed && !force64;
     if (restart) {
-      // Fallback: FP64 residual, p = r, every tile FP64 from here on.
-      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, false, wl)};
+      // FP64 residual, p = r; after a fallback every tile is FP64.
+      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, false, wl, lowv)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
       rr = tr[0];
@@ -221,11 +249,19 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
     double pq[1] = {0.0};
     if (infer && !force64) wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
     infer = false;
-    wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
-      [&](const int i, const double qv) {
-        a.q[i] = qv;
-        pq[0] += a.p[i] * qv;
-      });
+    if (lowv) {
+      wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, 2, wl[threadIdx.x >> 6], lane, &cnt,
+        [&](const int i, const double qv) {
+          a.q32[i] = (float)qv;
+          pq[0] += (double)a.p32[i] * qv;
+        });
+    } else {
+      wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
+        [&](const int i, const double qv) {
+          a.q[i] = qv;
+          pq[0] += a.p[i] * qv;
+        });
+    }
     grid_sum<1>(pq, a.partials, a.gs, epoch, slot, lds, &lds_i);
     if (mixed && !force64 && !(pq[0] > 0.0 && isfinite(pq[0]))) {
       // The low-precision operator lost positive definiteness.
@@ -237,12 +273,21 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
     const double alpha = rr / pq[0];
     // x += alpha p, r -= alpha q, ||r||^2.
     double rn[1] = {0.0};
-    for (int i = row0 + lane; i < row1; i += 64) {
-      const double pv = a.p[i];
-      a.x[i] += alpha * pv;
-      const double rv = a.r[i] - alpha * a.q[i];
-      a.r[i] = rv;
-      rn[0] += rv * rv;
+    if (lowv) {
+      for (int i = row0 + lane; i < row1; i += 64) {
+        a.x[i] += alpha * (double)a.p32[i];
+        const float rf = (float)((double)a.r32[i] - alpha * (double)a.q32[i]);
+        a.r32[i] = rf;
+        rn[0] += (double)rf * rf;
+      }
+    } else {
+      for (int i = row0 + lane; i < row1; i += 64) {
+        const double pv = a.p[i];
+        a.x[i] += alpha * pv;
+        const double rv = a.r[i] - alpha * a.q[i];
+        a.r[i] = rv;
+        rn[0] += rv * rv;
+      }
     }
     grid_sum<1>(rn, a.partials, a.gs, epoch, slot, lds, &lds_i);
     double rr_new = rn[0];
@@ -255,7 +300,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       if (converged || periodic) {
         // FP64 recomputation of the residual (x is complete: the grid sum
         // above is a barrier after the x update).
-        double tr[1] = {true_residual_rows(a, tr0, tr1, lane, a.r, wl)};
+        double tr[1] = {lowv ? true_residual_rows(a, tr0, tr1, lane, nullptr, wl, a.r32)
+                             : true_residual_rows(a, tr0, tr1, lane, a.r, wl)};
         grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
         ++refresh;
         it_ref = it;
@@ -284,10 +330,18 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       const int i = base + lane;
       double d = 0.0;
       if (i < row1) {
-        const double po = a.p[i];
-        const double pv = a.r[i] + beta * po;
-        a.p[i] = pv;
-        if (mixed) a.p32[i] = (float)pv;
+        double pv, po;
+        if (lowv) {
+          po = (double)a.p32[i];
+          const float pf = (float)((double)a.r32[i] + beta * po);
+          a.p32[i] = pf;
+          pv = pf;
+        } else {
+          po = a.p[i];
+          pv = a.r[i] + beta * po;
+          a.p[i] = pv;
+          if (mixed) a.p32[i] = (float)pv;
+        }
         d = lupd ? pv * pv : fabs(pv - po);
       }
       if (upd) {
@@ -828,7 +882,7 @@ namespace {
 
 struct Workspace {
   double *r, *p, *q, *rh, *s, *tv;
-  float *p32, *s32;
+  float *p32, *s32, *r32, *q32;
   uint8_t *seg_p, *seg_s;
   float *seg2;
   double *partials;
@@ -842,7 +896,7 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 // Carves the handle's device workspace (grown on demand).
 mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
-  size_t need = 6 * align256(n * 8) + 2 * align256(n * 4) + 2 * align256(segs) + align256(segs * 4) +
+  size_t need = 6 * align256(n * 8) + 4 * align256(n * 4) + 2 * align256(segs) + align256(segs * 4) +
                 align256(2 * kMaxGridSum * (size_t)G * 8) + align256(sizeof(GridSync)) + align256(kStCount * 8) +
                 align256(5 * 8);
   if (t->work_bytes < need) {
@@ -862,6 +916,8 @@ mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   w.tv = (double *)take(n * 8);
   w.p32 = (float *)take(n * 4);
   w.s32 = (float *)take(n * 4);
+  w.r32 = (float *)take(n * 4);
+  w.q32 = (float *)take(n * 4);
   w.seg_p = (uint8_t *)take(segs);
   w.seg_s = (uint8_t *)take(segs);
   w.seg2 = (float *)take(segs * 4);
@@ -1007,7 +1063,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   // Padding entries of the work vectors must be zero: tiles whose columns
   // lie beyond the rows gather them.
   MF_HIP(hipMemsetAsync(w.r, 0, 6 * align256(n * 8), st));
-  MF_HIP(hipMemsetAsync(w.p32, 0, 2 * align256(n * 4), st));
+  MF_HIP(hipMemsetAsync(w.p32, 0, 4 * align256(n * 4), st));
   if (opt.policy == MF_POLICY_FIXED)
     MF_HIP(hipMemcpyAsync(t->dev.prec, t->dev.prec_user, t->tilenum, hipMemcpyDeviceToDevice, st));
   if (opt.policy == MF_POLICY_ERRBOUND && t->tilenum > 0) {
@@ -1021,7 +1077,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   a.b = d_b;
   a.x = d_x;
   a.r = w.r; a.p = w.p; a.q = w.q; a.rh = w.rh; a.s = w.s; a.tv = w.tv;
-  a.p32 = w.p32; a.s32 = w.s32;
+  a.p32 = w.p32; a.s32 = w.s32; a.r32 = w.r32; a.q32 = w.q32;
   a.seg_p = w.seg_p; a.seg_s = w.seg_s;
   a.wave_tr = t->sched.wave_tr;
   a.partials = w.partials;
