@@ -117,7 +117,14 @@ mf_status mf_tile_arrays_get(const mf_tiled *t, mf_tile_arrays *out);
  *   mf_tile_balance: the reference's two-pointer nnz balancer, returns the
  *     warp count in *nwarps; balance_tile_ptr (nwarps+1, sized tilenum+2 is
  *     always enough), index_each_block / row_each_block (tilenum+1);
- *     replaces src/main-cg.cu:596-761. */
+ *     replaces src/main-cg.cu:596-761.
+ *   mf_tile_shared_offsets: from mf_tile_balance's output, per warp the
+ *     prefix of its tiles whose nnz fit a 256-entry LDS queue:
+ *     non_each_block_offset (tilenum+1; offset of each such tile inside the
+ *     queue, 0 for the others, which the reference leaves unset) and
+ *     balance_tile_ptr_shared_end (nwarps+1; end of the prefix, entry w+1
+ *     for warp w, the warp's own end when everything fits); replaces
+ *     src/main-cg-mixed.cu:2681-2708. */
 mf_status mf_tile_rowblkblock(const mf_tiled *t, int *rowblkblock);
 mf_status mf_tile_offsets(const mf_tiled *t, int *ptroffset1, int *ptroffset2,
                           unsigned int *blkcoostylerowidx, int *colstart,
@@ -131,6 +138,11 @@ mf_status mf_tile_compact_fill(const mf_tiled *t, const int *nonzero_row_new,
 mf_status mf_tile_balance(const mf_tiled *t, int warp_limit,
                           int *balance_tile_ptr, int *index_each_block,
                           int *row_each_block, int *nwarps);
+mf_status mf_tile_shared_offsets(const mf_tiled *t, int nwarps,
+                                 const int *balance_tile_ptr,
+                                 const int *index_each_block,
+                                 int *non_each_block_offset,
+                                 int *balance_tile_ptr_shared_end);
 
 /* Nibble codec for tile-local column indices, two per byte, even position in
  * the high nibble (src/encode.h:29-72). */

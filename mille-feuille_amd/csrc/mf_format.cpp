@@ -609,6 +609,40 @@ int balance_sweep(int T, const std::vector<int> &nb, int E, int B, int *btp, int
 }
 }  // namespace
 
+extern "C" mf_status mf_tile_shared_offsets(const mf_tiled *t, int nwarps,
+                                            const int *balance_tile_ptr,
+                                            const int *index_each_block,
+                                            int *non_each_block_offset,
+                                            int *balance_tile_ptr_shared_end) {
+  if (!t || !balance_tile_ptr || !index_each_block || !non_each_block_offset ||
+      !balance_tile_ptr_shared_end || nwarps < 0)
+    return fail(MF_ERR_ARG, "mf_tile_shared_offsets: bad argument");
+  const int T = t->tilenum;
+  constexpr int kSharedNnz = 256;  // LDS budget per warp in entries
+  for (int w = 0; w < nwarps; ++w)
+    if (balance_tile_ptr[w] < 0 || balance_tile_ptr[w] > balance_tile_ptr[w + 1] ||
+        balance_tile_ptr[w + 1] > T)
+      return fail(MF_ERR_ARG, "mf_tile_shared_offsets: warp bounds outside [0, tilenum]");
+  std::fill(non_each_block_offset, non_each_block_offset + T + 1, 0);
+  for (int w = 0; w <= nwarps; ++w) balance_tile_ptr_shared_end[w] = balance_tile_ptr[w];
+  for (int w = 0; w < nwarps; ++w) {
+    // the warp's tiles in balancer order while their running nnz fits
+    int run = 0;
+    for (int j = balance_tile_ptr[w]; j < balance_tile_ptr[w + 1]; ++j) {
+      const int tj = index_each_block[j];
+      if (tj < 0 || tj >= T) return fail(MF_ERR_ARG, "mf_tile_shared_offsets: tile index out of range");
+      const int nz = t->blknnz[tj + 1] - t->blknnz[tj];
+      run += nz;
+      if (run > kSharedNnz) {
+        balance_tile_ptr_shared_end[w + 1] = j;
+        break;
+      }
+      non_each_block_offset[j] = run - nz;
+    }
+  }
+  return MF_OK;
+}
+
 extern "C" mf_status mf_tile_balance(const mf_tiled *t, int warp_limit, int *btp,
                                      int *index_each_block, int *row_each_block,
                                      int *nwarps) {

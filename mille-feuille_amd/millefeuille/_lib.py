@@ -103,6 +103,7 @@ SIGNATURES = {
     "mf_tile_offsets": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "mf_tile_compact_count": (C.c_int, [_vp, _vp, _ip]),
     "mf_tile_compact_fill": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "mf_tile_shared_offsets": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
     "mf_tile_balance": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _ip]),
     "mf_encode_nibbles": (None, [_vp, _vp, C.c_int, C.c_int]),
     "mf_decode_nibbles": (None, [_vp, _vp, C.c_int]),

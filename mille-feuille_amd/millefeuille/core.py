@@ -141,6 +141,17 @@ class TiledMatrix:
         return {"nwarps": nw.value, "balance_tile_ptr": btp[: nw.value + 1],
                 "index_each_block": ieb, "row_each_block": reb}
 
+    def shared_offsets(self, bal):
+        """The reference's shared-queue offsets (mf_tile_shared_offsets) of a
+        balance() result."""
+        nw = int(bal["nwarps"])
+        btp = np.ascontiguousarray(bal["balance_tile_ptr"], np.int32)
+        ieb = np.ascontiguousarray(bal["index_each_block"], np.int32)
+        off = np.zeros(self.tilenum + 1, np.int32)
+        end = np.zeros(nw + 1, np.int32)
+        check("mf_tile_shared_offsets", lib.mf_tile_shared_offsets(self.h, nw, _p(btp), _p(ieb), _p(off), _p(end)))
+        return {"non_each_block_offset": off, "balance_tile_ptr_shared_end": end}
+
     # -- precision ---------------------------------------------------------
     def set_precision(self, levels=None, uniform=0):
         lv = None if levels is None else np.ascontiguousarray(levels, dtype=np.uint8)

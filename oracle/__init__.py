@@ -46,6 +46,7 @@ for name, res, args in [
     ("or_compact_count", _I, [_I, _I, _v, _v, _v, _v]),
     ("or_compact_fill", None, [_I, _I, _v, _v, _v, _v, _v, _v, _v, _v]),
     ("or_load_balance", _I, [_I, _I, _I, _v, _v, _I, _v, _v, _v, _ip, _ip]),
+    ("or_shared_offsets", None, [_I, _v, _I, _v, _v, _v, _v]),
     ("or_cg", _I, [_I, _v, _v, _v, _v, _v, C.c_double, _I, C.POINTER(C.c_double)]),
     ("or_bicgstab", _I, [_I, _v, _v, _v, _v, _v, C.c_double, _I, C.POINTER(C.c_double)]),
     ("or_rel_residual", C.c_double, [_I, _v, _v, _v, _v, _v]),
@@ -205,6 +206,17 @@ def load_balance(t, nnzR, warp_limit=6912):
         return None
     return {"nwarps": idx, "balance_tile_ptr": btp[: idx + 1], "index_each_block": ieb,
             "row_each_block": reb, "each_block_nnz": E.value, "block_per_warp": B.value}
+
+
+def shared_offsets(t, bal):
+    """src/main-cg-mixed.cu:2681-2708 restated: shared-queue offsets of a
+    load_balance() result."""
+    T, nw = t["tilenum"], int(bal["nwarps"])
+    off = np.zeros(T + 1, np.int32)
+    end = np.zeros(nw + 1, np.int32)
+    _o.or_shared_offsets(T, _p(_i32(t["blknnz"])), nw, _p(_i32(bal["balance_tile_ptr"])),
+                         _p(_i32(bal["index_each_block"])), _p(off), _p(end))
+    return {"non_each_block_offset": off, "balance_tile_ptr_shared_end": end}
 
 
 # -- solvers ------------------------------------------------------------------

@@ -603,6 +603,37 @@ int or_load_balance(int tilem, int tilenum, int nnzR, const int *tile_ptr,
   return idx2;
 }
 
+/* Shared-queue offsets of the balancer's warps, src/main-cg-mixed.cu:2681-2708:
+ * per warp, walk its tiles (balancer order) accumulating nnz; a tile whose
+ * running total stays <= 256 gets the previous tile's offset + nnz (the
+ * warp's first tile 0); the first tile that overflows ends the warp's shared
+ * prefix.  Entries the reference never writes (malloc'd) are left 0 here. */
+void or_shared_offsets(int tilenum, const int *blknnz, int nwarps,
+                       const int *balance_tile_ptr, const int *index_each_block,
+                       int *non_each_block_offset, int *shared_end) {
+  int *nb_new = (int *)malloc(sizeof(int) * (size_t)(tilenum + 1));
+  for (int j = 0; j < tilenum; ++j) {
+    const int t = index_each_block[j];
+    nb_new[j] = blknnz[t + 1] - blknnz[t];
+  }
+  memset(non_each_block_offset, 0, sizeof(int) * (size_t)(tilenum + 1));
+  for (int i = 0; i <= nwarps; ++i) shared_end[i] = balance_tile_ptr[i];
+  for (int i = 0; i < nwarps; ++i) {
+    int cnt_nnz = 0;
+    for (int j = balance_tile_ptr[i]; j < balance_tile_ptr[i + 1]; ++j) {
+      if (j == balance_tile_ptr[i]) non_each_block_offset[j] = 0;
+      cnt_nnz += nb_new[j];
+      if (j != balance_tile_ptr[i] && cnt_nnz <= 256)
+        non_each_block_offset[j] = nb_new[j - 1] + non_each_block_offset[j - 1];
+      if (cnt_nnz > 256) {
+        shared_end[i + 1] = j;
+        break;
+      }
+    }
+  }
+  free(nb_new);
+}
+
 /* ------------------------------------------------------------------------ */
 /* FP64 solvers                                                              */
 /* ------------------------------------------------------------------------ */

@@ -33,6 +33,14 @@
  *                                  balancer lives in .cu host code that cannot
  *                                  be compiled here -> PARITY UNPINNED beyond
  *                                  those invariants.
+ *   or_shared_offsets              the closed form (per warp, exclusive prefix
+ *                                  sums of its tiles' nnz up to the first
+ *                                  tile that overflows 256; that tile ends the
+ *                                  prefix) recomputed in numpy on every
+ *                                  fixture and on hand-built warps at the 256
+ *                                  boundary; same CUDA-host-code limitation
+ *                                  as or_load_balance -> PARITY UNPINNED
+ *                                  against the reference's own run.
  *   or_cg/or_bicgstab              scipy.sparse.linalg on the same systems
  *                                  (test-time only) and the closed-form answer
  *                                  x = ones for b = A*ones.
@@ -111,6 +119,9 @@ void or_compact_fill(int rowA, int tilem, const int *tile_ptr,
                      int *tile_rowidx);
 
 /* ---- nnz load balancer (src/main-cg.cu:596-740) ---- */
+void or_shared_offsets(int tilenum, const int *blknnz, int nwarps,
+                       const int *balance_tile_ptr, const int *index_each_block,
+                       int *non_each_block_offset, int *shared_end);
 int or_load_balance(int tilem, int tilenum, int nnzR, const int *tile_ptr,
                     const int *blknnz, int warp_limit, int *balance_tile_ptr,
                     int *index_each_block, int *row_each_block,

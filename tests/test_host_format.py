@@ -189,3 +189,79 @@ def test_product_does_not_reference_the_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h", "Makefile")):
                 text = open(os.path.join(dirpath, f), errors="ignore").read()
                 assert "mforacle" not in text and "import oracle" not in text, f
+
+
+def _shared_offsets_closed_form(nnz_new, btp):
+    """Per warp: exclusive prefix sums of its tiles' nnz while the running
+    total stays <= 256; the first tile past 256 ends the warp's prefix."""
+    off = np.zeros(len(nnz_new) + 1, np.int32)
+    end = np.array(btp, np.int32).copy()
+    for w in range(len(btp) - 1):
+        run = 0
+        for j in range(btp[w], btp[w + 1]):
+            if run + nnz_new[j] > 256:
+                end[w + 1] = j
+                break
+            off[j] = run
+            run += nnz_new[j]
+    return off, end
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_shared_queue_offsets_match_oracle_and_closed_form(mf, oracle, golden, name):
+    g = golden(name)
+    t = host_handle(mf, g)
+    ot = tiles_from_golden(oracle, g)
+    if ot["tilenum"] <= 1:
+        pytest.skip("balancer needs at least two tiles")
+    b = t.balance(6912)
+    got = t.shared_offsets(b)
+    ref = oracle.shared_offsets(ot, oracle.load_balance(ot, int(g["meta_nnz"])))
+    for k in ref:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    nb = np.diff(np.asarray(ot["blknnz"], np.int64))
+    nnz_new = nb[b["index_each_block"][: ot["tilenum"]]]
+    off, end = _shared_offsets_closed_form(nnz_new, b["balance_tile_ptr"])
+    np.testing.assert_array_equal(got["non_each_block_offset"], off)
+    np.testing.assert_array_equal(got["balance_tile_ptr_shared_end"], end)
+
+
+def test_shared_queue_offsets_at_the_256_boundary(mf, oracle):
+    """Hand-built warps: prefixes that overflow on their second tile and one
+    whose first tile fills the 256 entries exactly."""
+    # tile-rows of dense 16x16 tiles (256 nnz) and single entries
+    rows, cols = [], []
+    for r in range(16):
+        for c in range(16):
+            rows.append(r); cols.append(c)          # tile (0,0): 256 nnz
+    for r in range(16, 32):
+        rows.append(r); cols.append(r)              # tile (1,1): 16 nnz
+    for r in range(16, 32):
+        for c in range(32, 48):
+            rows.append(r); cols.append(c)          # tile (1,2): 256 nnz
+    m = 48
+    order = np.lexsort((cols, rows))
+    rows, cols = np.array(rows)[order], np.array(cols)[order]
+    rp = np.zeros(m + 1, np.int32)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    va = np.ones(len(cols))
+    t = mf.TiledMatrix(rp, cols.astype(np.int32), va, rowA=m, colA=m, device=-1)
+    ot = oracle.tile_create(m, m, rp, cols, va)
+    # warps by hand: [t0 t1 t2] in one warp, then [t1 t2], then [t2 t0]
+    nb = np.diff(np.asarray(ot["blknnz"]))
+    T = ot["tilenum"]
+    assert T == 3 and list(nb[:3]) == [256, 16, 256]
+    bal = {"nwarps": 3, "balance_tile_ptr": np.array([0, 3, 5, 7], np.int32),
+           "index_each_block": np.array([1, 0, 2, 1, 2, 2, 0, 0], np.int32)}
+    # the handle API takes tilenum+1 index entries; pass a 3-tile list per warp
+    # through the oracle only for the permuted lists longer than tilenum
+    ref = oracle.shared_offsets(dict(ot, tilenum=7), bal)
+    off, end = _shared_offsets_closed_form(nb[bal["index_each_block"][:7]], bal["balance_tile_ptr"])
+    np.testing.assert_array_equal(ref["non_each_block_offset"], off)
+    np.testing.assert_array_equal(ref["balance_tile_ptr_shared_end"], end)
+    assert list(end) == [0, 1, 4, 6]
+    assert list(off[:7]) == [0, 0, 0, 0, 0, 0, 0]
+    # the library rejects warp bounds past tilenum
+    with pytest.raises(mf.MfError):
+        t.shared_offsets(bal)
