@@ -7,7 +7,8 @@ nnz = 14,581,760, 880,640 16x16 tiles), diagonal 6*10^(3U) with U ~ U[0,1)
 from seed 2, b ~ U[-1,1] from seed 2, x0 = 0, tolerance 1e-10.
 
 A step is one complete solve (x reset to 0, one persistent-kernel launch, to
-the tolerance) with the matrix and b already resident in HBM.  `value` is the
+the tolerance) with the matrix and b already resident in HBM; the mixed and
+FP64 solves alternate step by step (each timed on its own).  `value` is the
 mixed-precision solve time in ms (lower is better).  Also reported: the FP64
 single-kernel CG time on the same matrix, the speedup, iterations, the FP64
 residual recomputed on the CPU, the per-tile precision histogram, the SpMV
@@ -45,6 +46,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--policy", default="learned")
+    ap.add_argument("--check-interval", type=int, default=32,
+                    help="iterations between policy evaluations (learned inference)")
     ap.add_argument("--cpu-iters", type=int, default=0,
                     help="CPU baseline CG iterations (0: run it to the tolerance)")
     ap.add_argument("--spmv-runs", type=int, default=50)
@@ -72,38 +75,39 @@ def main():
     if policy == "learned":
         from millefeuille import policy as pol
         pol.install(tm, pol.load_or_train())
-        policy_note = "learned (MLP 8-32-32-4, policy_mlp.f32)"
+        policy_note = (f"learned (MLP 8-32-32-4, policy_mlp.f32, in-kernel inference every "
+                       f"{args.check_interval} iterations)")
 
     def solve(pol):
         dx.zero()
-        return tm.cg(db, dx, tol=1e-10, policy=pol)
+        return tm.cg(db, dx, tol=1e-10, policy=pol, check_interval=args.check_interval)
 
-    # --- warm-up and timed mixed solves ---------------------------------
+    # --- warm-up, then mixed and FP64 solves alternated step by step, so
+    # both see the same device state; each solve timed on its own --------
+    import oracle
     for _ in range(args.warmup):
         solve(policy)
+        solve("fp64")
     mf.synchronize()
-    t1 = time.perf_counter()
-    reps = [solve(policy) for _ in range(args.steps)]
-    mf.synchronize()
-    mixed_ms = (time.perf_counter() - t1) * 1e3 / args.steps
-    rep = reps[-1]
-    x = dx.get()[:n]
-    import oracle
+    t_mixed, t_fp64 = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        rep = solve(policy)
+        t_mixed.append(time.perf_counter() - t1)
+        x = dx.get()[:n]
+        t2 = time.perf_counter()
+        rep64 = solve("fp64")
+        t_fp64.append(time.perf_counter() - t2)
+    x64 = dx.get()[:n]
+    mixed_ms = 1e3 * sum(t_mixed) / args.steps
+    fp64_ms = 1e3 * sum(t_fp64) / args.steps
     rel_cpu = oracle.rel_residual(rp, ci, va, b, x)
+    rel64 = oracle.rel_residual(rp, ci, va, b, x64)
     log(f"mixed[{policy}]: {mixed_ms:.2f} ms/solve (kernel {rep['time_ms']:.2f} ms), it={rep['iterations']} "
         f"refresh={rep['refreshes']} fallback={rep['fell_back']} policy_used={rep['policy_used']} "
         f"rel(cpu)={rel_cpu:.3e} hist={rep['prec_hist']}")
-
-    # --- FP64 single-kernel CG on the same matrix --------------------------
-    solve("fp64")
-    mf.synchronize()
-    t2 = time.perf_counter()
-    reps64 = [solve("fp64") for _ in range(args.steps)]
-    mf.synchronize()
-    fp64_ms = (time.perf_counter() - t2) * 1e3 / args.steps
-    rep64 = reps64[-1]
-    rel64 = oracle.rel_residual(rp, ci, va, b, dx.get()[:n])
-    log(f"fp64: {fp64_ms:.2f} ms/solve, it={rep64['iterations']} rel(cpu)={rel64:.3e}")
+    log(f"fp64: {fp64_ms:.2f} ms/solve (kernel {rep64['time_ms']:.2f} ms), it={rep64['iterations']} "
+        f"rel(cpu)={rel64:.3e}")
 
     # --- stand-alone SpMV roofline (all-FP64 payload) ----------------------
     xs = np.random.default_rng(0).uniform(-1, 1, n)

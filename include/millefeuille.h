@@ -228,7 +228,7 @@ typedef struct mf_solve_opts {
   int fixed_iters;      /* >0: run exactly this many iterations, no test
                            (reference timing mode, src/Mille-feuille-cg.h:426) */
   int form;             /* CG kernel form: MF_FORM_AUTO, MF_FORM_MEMORY (vectors
-                           in HBM, three syncs per iteration) or MF_FORM_RESIDENT
+                           in HBM, two syncs per iteration) or MF_FORM_RESIDENT
                            (vectors in registers/LDS, two syncs; needs regular
                            tile-rows and at most 8192 rows per CU) */
   int batch_waves;      /* mf_batch_cg_solve: waves per (system, right-hand

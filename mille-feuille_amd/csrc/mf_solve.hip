@@ -175,6 +175,14 @@ __device__ __forceinline__ double restart_rows(const SolveArgs &a, int tr0, int 
   return acc;
 }
 
+// Memory-form CG in the Chronopoulos-Gear arrangement: one SpMV w = A r whose
+// epilogue accumulates gamma = r.r and delta = w.r, one fused grid sum, then
+// p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s and a grid sync:
+// two grid-wide synchronisations per iteration.  With a mixed policy r, w,
+// p and s are FP32 (x, the scalars and every residual recomputation stay
+// FP64; FP64 tiles multiply the FP32 r in FP64 arithmetic); after a
+// fallback to FP64 tiles the recurrence continues in FP64 vectors.
+// Vectors: FP64 r, q (w), p, s / FP32 r32, q32 (w), p32, s32.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   __shared__ double lds[kWavesPerBlock];
@@ -190,18 +198,15 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   int slot = 0;
   PrecCount cnt;
 
-  // r = b - A x0, p = r (FP64, and the FP32 copies the mixed recurrence
-  // starts from).
+  // r = b - A x0 (FP64, and the FP32 copy the mixed recurrence starts from)
   double red[2] = {0.0, 0.0};
   wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
       const double bv = a.b[i];
       double rv = bv - ax;
       a.r[i] = rv;
-      a.p[i] = rv;
       if (mixed) {
         const float rf = (float)rv;
-        a.p32[i] = rf;
         a.r32[i] = rf;
         rv = rf;
       }
@@ -209,11 +214,11 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       red[1] += bv * bv;
     });
   if (learned) {
-    // segment norms of p = r0 for the first inference
+    // segment norms of r0 for the first inference
     for (int base = row0; base < row1; base += 64) {
       const int i = base + lane;
-      const double pv = i < row1 ? (double)a.p32[i] : 0.0;
-      const double d2 = group16_sum(pv * pv);
+      const double rv = i < row1 ? (double)a.r32[i] : 0.0;
+      const double d2 = group16_sum(rv * rv);
       if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d2;
     }
   }
@@ -221,146 +226,164 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   double rr = red[0];
   const double bb = red[1];
   const double stop2 = a.tol2 * bb;
-  double rr_ref = rr;  // residual norm^2 at the last FP64 recomputation
+  double rr_ref = rr, gamma_old = 1.0, alpha_old = 1.0;
+  bool first = true;
   int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0;
-  bool restart = false;
   bool done = bb == 0.0 || (a.fixed_iters <= 0 && rr <= stop2);
+  bool pending = false;  // recompute the FP64 residual before the next step
   bool infer = learned;  // learned policy: new levels in the next SpMV
-  double pp_last = rr;   // ||p||^2 at the last inference point
+  bool lowv = mixed;     // FP32 vectors (until a fallback to FP64 tiles)
 
-  // Mixed policies keep r, p and q in FP32 (x, the scalars and the
-  // residual recomputations stay FP64); after a fallback to FP64 tiles the
-  // recurrence restarts in FP64 vectors.
   while (!done) {
-    const bool lowv = mixed && !force64;
-    if (restart) {
-      // FP64 residual, p = r; after a fallback every tile is FP64.
-      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, false, wl, lowv)};
+    double gamma = 0.0, delta = 0.0;
+    if (!pending) {
+      // w = A r with gamma = r.r and delta = w.r of the wave's rows
+      if (infer && !force64) wave_infer(a.A, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
+      infer = false;
+      double gd[2] = {0.0, 0.0};
+      wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.r, a.r32, a.seg_p, lowv ? 2 : force64, wl[threadIdx.x >> 6],
+                             lane, &cnt, [&](const int i, const double wv) {
+        double rv;
+        if (lowv) {
+          rv = a.r32[i];
+          a.q32[i] = (float)wv;
+        } else {
+          rv = a.r[i];
+          a.q[i] = wv;
+        }
+        gd[0] += rv * rv;
+        gd[1] += wv * rv;
+      });
+      grid_sum<2>(gd, a.partials, a.gs, epoch, slot, lds, &lds_i);
+      gamma = gd[0];
+      delta = gd[1];
+      rr = gamma;
+      if (a.fixed_iters <= 0) {
+        const bool converged = gamma <= stop2;
+        const bool periodic = mixed && !force64 &&
+                              (gamma <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
+        pending = converged || periodic;
+        if (!pending && it >= a.maxiter) { status = MF_NOT_CONVERGED; break; }
+      }
+    }
+    if (pending) {
+      pending = false;
+      // FP64 residual replacement r = b - A x (x is complete: the last
+      // update ended with a grid sync)
+      const bool lv = lowv && !force64;
+      double tr[1] = {lv ? true_residual_rows(a, tr0, tr1, lane, nullptr, wl, a.r32)
+                         : true_residual_rows(a, tr0, tr1, lane, a.r, wl)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
-      rr = tr[0];
-      rr_ref = rr;
       it_ref = it;
-      restart = false;
-      if (rr <= stop2) break;
-    }
-    ++it;
-    // q = A p, fused with p.q for the wave's rows.
-    double pq[1] = {0.0};
-    if (infer && !force64) wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
-    infer = false;
-    if (lowv) {
-      wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, 2, wl[threadIdx.x >> 6], lane, &cnt,
-        [&](const int i, const double qv) {
-          a.q32[i] = (float)qv;
-          pq[0] += (double)a.p32[i] * qv;
-        });
-    } else {
-      wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
-        [&](const int i, const double qv) {
-          a.q[i] = qv;
-          pq[0] += a.p[i] * qv;
-        });
-    }
-    grid_sum<1>(pq, a.partials, a.gs, epoch, slot, lds, &lds_i);
-    if (mixed && !force64 && !(pq[0] > 0.0 && isfinite(pq[0]))) {
-      // The low-precision operator lost positive definiteness.
-      force64 = 1;
-      restart = true;
+      // A recursive residual far below the true one leaves beta meaningless,
+      // and so does a recursive residual that met the tolerance while the
+      // true one did not (the s = A p recurrence drifted): restart the
+      // search direction from the recomputed residual.
+      if (tr[0] > 4.0 * rr || (rr <= stop2 && tr[0] > stop2)) first = true;
+      rr = tr[0];
+      if (tr[0] <= stop2) break;
+      if (it >= a.maxiter) { status = MF_NOT_CONVERGED; break; }
+      if (mixed && !force64 && (tr[0] > 0.25 * rr_ref || refresh > a.max_refresh)) {
+        // stagnation: FP64 tiles (and FP64 vectors) from here, restart
+        force64 = 1;
+        first = true;
+      }
+      if (lowv && force64) {
+        // leaving the FP32 vectors (force64 implies a restart, first):
+        // the FP64 recurrence starts from an FP64 residual
+        lowv = false;
+        if (lv) {
+          double t2[1] = {true_residual_rows(a, tr0, tr1, lane, a.r, wl)};
+          grid_sum<1>(t2, a.partials, a.gs, epoch, slot, lds, &lds_i);
+          rr = t2[0];
+        }
+      }
+      rr_ref = rr;
+      if (!first) {
+        // s = A p in FP64 so that s and the new r stay consistent
+        if (lowv) {
+          for (int i = row0 + lane; i < row1; i += 64) a.p[i] = (double)a.p32[i];
+          grid_sync(a.gs, epoch, &lds_i);
+          wave_spmv<kAll64, false>(a.A, tr0, tr1, a.p, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane,
+                                   nullptr, [&](const int i, const double sv) { a.s32[i] = (float)sv; });
+        } else {
+          wave_spmv<kAll64, false>(a.A, tr0, tr1, a.p, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane,
+                                   nullptr, [&](const int i, const double sv) { a.s[i] = sv; });
+        }
+      }
+      grid_sync(a.gs, epoch, &lds_i);
       continue;
     }
-    if (!(pq[0] != 0.0) || !isfinite(pq[0])) { status = MF_ERR_BREAKDOWN; break; }
-    const double alpha = rr / pq[0];
-    // x += alpha p, r -= alpha q, ||r||^2.
-    double rn[1] = {0.0};
-    if (lowv) {
-      for (int i = row0 + lane; i < row1; i += 64) {
-        a.x[i] += alpha * (double)a.p32[i];
-        const float rf = (float)((double)a.r32[i] - alpha * (double)a.q32[i]);
-        a.r32[i] = rf;
-        rn[0] += (double)rf * rf;
-      }
+    double beta, alpha;
+    if (first) {
+      beta = 0.0;
+      alpha = gamma / delta;
     } else {
-      for (int i = row0 + lane; i < row1; i += 64) {
-        const double pv = a.p[i];
-        a.x[i] += alpha * pv;
-        const double rv = a.r[i] - alpha * a.q[i];
-        a.r[i] = rv;
-        rn[0] += rv * rv;
-      }
+      beta = gamma / gamma_old;
+      alpha = gamma / (delta - beta * gamma / alpha_old);
     }
-    grid_sum<1>(rn, a.partials, a.gs, epoch, slot, lds, &lds_i);
-    double rr_new = rn[0];
-    if (a.fixed_iters > 0) {
-      if (it >= a.fixed_iters) { rr = rr_new; break; }
-    } else {
-      const bool converged = rr_new <= stop2;
-      const bool periodic = mixed && !force64 &&
-                            (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
-      if (converged || periodic) {
-        // FP64 recomputation of the residual (x is complete: the grid sum
-        // above is a barrier after the x update).
-        double tr[1] = {lowv ? true_residual_rows(a, tr0, tr1, lane, nullptr, wl, a.r32)
-                             : true_residual_rows(a, tr0, tr1, lane, a.r, wl)};
-        grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
-        ++refresh;
-        it_ref = it;
-        if (tr[0] <= stop2) { rr = tr[0]; break; }
-        // No progress since the last recomputation, or too many refinement
-        // steps: every tile back to FP64 and restart the recurrence.
-        if (mixed && !force64 && (tr[0] > 0.25 * rr_ref || refresh > a.max_refresh)) {
-          force64 = 1;
-          restart = true;
-        }
-        // A recursive residual far below the true one makes beta meaningless.
-        if (tr[0] > 4.0 * rr_new) restart = true;
-        rr_new = tr[0];
-        rr_ref = tr[0];
+    if (!isfinite(alpha) || alpha == 0.0 || (mixed && !force64 && !(alpha > 0.0))) {
+      if (mixed && !force64) {
+        // The low-precision operator lost definiteness: FP64 tiles,
+        // recompute r = b - A x, restart.
+        force64 = 1;
+        first = true;
+        pending = true;
+        continue;
       }
-      if (it >= a.maxiter) { rr = rr_new; status = MF_NOT_CONVERGED; break; }
-      if (restart) continue;
+      status = MF_ERR_BREAKDOWN;
+      break;
     }
-    const double beta = rr_new / rr;
-    rr = rr_new;
-    // p = r + beta p; the rule policy looks at |p_new - p_old| per segment.
+    first = false;
+    gamma_old = gamma;
+    alpha_old = alpha;
+    ++it;
+    // p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s; the rule
+    // policy looks at |r_new - r_old| per segment, the learned one records
+    // the segment norms of r_new for its next inference.
     const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
     const bool lupd = learned && !force64 && (it % a.check_interval) == 0;
-    double pp[1] = {0.0};
     for (int base = row0; base < row1; base += 64) {
       const int i = base + lane;
       double d = 0.0;
       if (i < row1) {
-        double pv, po;
+        double rv, rn;
         if (lowv) {
-          po = (double)a.p32[i];
-          const float pf = (float)((double)a.r32[i] + beta * po);
+          rv = a.r32[i];
+          const double pv = rv + beta * (double)a.p32[i];
+          const double sv = (double)a.q32[i] + beta * (double)a.s32[i];
+          const float pf = (float)pv, sf = (float)sv;
           a.p32[i] = pf;
-          pv = pf;
+          a.s32[i] = sf;
+          a.x[i] += alpha * (double)pf;
+          const float rf = (float)(rv - alpha * (double)sf);
+          a.r32[i] = rf;
+          rn = rf;
         } else {
-          po = a.p[i];
-          pv = a.r[i] + beta * po;
+          rv = a.r[i];
+          const double pv = rv + beta * a.p[i];
+          const double sv = a.q[i] + beta * a.s[i];
           a.p[i] = pv;
-          if (mixed) a.p32[i] = (float)pv;
+          a.s[i] = sv;
+          a.x[i] += alpha * pv;
+          rn = rv - alpha * sv;
+          a.r[i] = rn;
         }
-        d = lupd ? pv * pv : fabs(pv - po);
+        d = lupd ? rn * rn : fabs(rn - rv);
       }
       if (upd) {
         d = group16_max(d);
         if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
       }
       if (lupd) {
-        pp[0] += d;
         d = group16_sum(d);
         if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d;
       }
     }
-    if (lupd) {
-      grid_sum<1>(pp, a.partials, a.gs, epoch, slot, lds, &lds_i);
-      pp_last = pp[0];
-      infer = true;
-    } else {
-      grid_sync(a.gs, epoch, &lds_i);
-    }
+    infer = lupd;
+    grid_sync(a.gs, epoch, &lds_i);
+    if (a.fixed_iters > 0 && it >= a.fixed_iters) break;
   }
   if (mixed) flush_hist(cnt, a.hist, lds);
   // Final FP64 residual for the report.
