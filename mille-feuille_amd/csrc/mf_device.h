@@ -261,7 +261,64 @@ __device__ __forceinline__ void chunk_gather_lv_fixed(const WaveLds &L,
   }
 }
 
-// Stage 3 for the fixed-width loads (per-tile levels).
+// Adds a chunk's four slot products into the window's row accumulators.
+// The slots of one tile are in CSR order, so slots of one row are adjacent:
+// each run of equal rows is summed in registers and added once, by the slot
+// that ends it.  Padding slots repeat the last entry's (row,col) byte and
+// carry a zero product, so they extend that entry's run.  No branches on the
+// slot data: the four adds are exec-masked.
+__device__ __forceinline__ void chunk_accumulate(WaveLds &L, int k0, int k1, int k2, int k3,
+                                                 double p0, double p1, double p2, double p3) {
+  const double s1 = p1 + (k1 == k0 ? p0 : 0.0);
+  const double s2 = p2 + (k2 == k1 ? s1 : 0.0);
+  const double s3 = p3 + (k3 == k2 ? s2 : 0.0);
+  if (k1 != k0) atomicAdd(&L.acc[k0], p0);
+  if (k2 != k1) atomicAdd(&L.acc[k1], s1);
+  if (k3 != k2) atomicAdd(&L.acc[k2], s2);
+  atomicAdd(&L.acc[k3], s3);
+}
+
+// Row accumulator index of slot u: the tile's row base plus the row nibble.
+__device__ __forceinline__ int chunk_row(const Chunk &ch, int ro, int u) {
+  return ro + ((ch.rc4 >> (8 * u + 4)) & 15);
+}
+
+// Stage 3 for the fixed-width loads (per-tile levels), without branches on
+// the slot data.  Every level's value is decoded and the tile's level picks
+// one: FP64 a*x in FP64 (x32only: a*(double)x32); FP32 (a*x32); FP16 and
+// INT8 (a*x32)*scale, rounded to FP32 then widened.  SKIP and invalid slots
+// contribute 0.
+__device__ __forceinline__ void chunk_consume_lv_flat(WaveLds &L, const Chunk &ch, bool x32only) {
+  const int ro = L.trow[ch.j] * 16;
+  const int lv = ch.lvl;
+  const float sc = lv == MF_PREC_FP16 || lv == MF_PREC_INT8 ? L.tscale[ch.j] : 1.0f;
+  const uint32_t w0 = ch.wsel == 0 ? ch.a.x : ch.wsel == 1 ? ch.a.y : ch.wsel == 2 ? ch.a.z : ch.a.w;
+  const uint32_t h0 = ch.wsel == 0 ? ch.a.x : ch.a.z;  // FP16: words 2*wsel, 2*wsel+1
+  const uint32_t h1 = ch.wsel == 0 ? ch.a.y : ch.a.w;
+  const int live = lv == MF_PREC_SKIP ? 0 : ch.vmask;
+  double p[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int col = (ch.rc4 >> (8 * u)) & 15;
+    const uint64_t raw = __builtin_bit_cast(uint64_t, chunk_x(ch, u));
+    const float xf = __builtin_bit_cast(float, (uint32_t)(col & 1 ? raw >> 32 : raw));
+    const double a64 = __builtin_bit_cast(double, ((uint64_t)chunk_word(ch, 2 * u + 1) << 32) |
+                                                      chunk_word(ch, 2 * u));
+    const double x64 = x32only ? (double)xf : __builtin_bit_cast(double, raw);
+    const float v32 = __builtin_bit_cast(float, chunk_word(ch, u));
+    const float v16 = half_bits_to_float((uint16_t)((u < 2 ? h0 : h1) >> (16 * (u & 1))));
+    const float v8 = (float)(int8_t)(w0 >> (8 * u));
+    const float vn = lv == MF_PREC_FP32 ? v32 : lv == MF_PREC_FP16 ? v16 : v8;
+    const double prod = lv == MF_PREC_FP64 ? a64 * x64 : (double)(vn * xf * sc);
+    p[u] = (live >> u & 1) ? prod : 0.0;
+  }
+  chunk_accumulate(L, chunk_row(ch, ro, 0), chunk_row(ch, ro, 1), chunk_row(ch, ro, 2),
+                   chunk_row(ch, ro, 3), p[0], p[1], p[2], p[3]);
+}
+
+// Stage 3 for the fixed-width loads (per-tile levels), branching per slot:
+// fewer live registers than chunk_consume_lv_flat, for kernels whose
+// register budget is shared with a solver loop.
 __device__ __forceinline__ void chunk_consume_lv_fixed(WaveLds &L, const Chunk &ch, bool x32only) {
   if (!ch.vmask || ch.lvl == MF_PREC_SKIP) return;
   const int ro = L.trow[ch.j] * 16;
@@ -402,11 +459,146 @@ __device__ __forceinline__ WaveDesc wave_desc(const TileView &A, int tr0, int tr
   d.T1 = A.tile_ptr[tr1];
   d.Z0 = A.tile_off[d.ta];
   d.Z1 = A.tile_off[d.T1];
-  d.pad0 = d.pad1 = 0;
+  d.flags = d.pad1 = 0;
   return d;
 }
 
-template <int MODE, bool COUNT, int DEPTH = 3, typename Epi>
+// Tile j of the window into LDS: column, tile-row, end of its real entries,
+// level and scale, and j in every chunk of its payload.  o, e: the tile's
+// payload begin and end relative to the window's first slot.
+template <int MODE, bool COUNT>
+__device__ __forceinline__ void stage_tile(WaveLds &L, int j, int o, int e, int col, int trow,
+                                           int nz, int lvl, float sc, PrecCount *cnt) {
+  L.tcol[j] = col << 4;
+  L.trow[j] = (uint8_t)trow;
+  L.tend[j] = o + nz + 1;
+  for (int q = o >> 2; q < (e >> 2); ++q) L.ctile[q] = (uint8_t)j;
+  if (MODE != kAll64) {
+    L.tprec[j] = (uint8_t)lvl;
+    L.tscale[j] = sc;
+    if (COUNT) cnt->add(lvl);
+  }
+}
+
+// The staged window's nch chunks from slot Z0 on, in slices of 64 chunks,
+// with the loads of later slices in flight while a slice is gathered and
+// consumed.  FLAT: per-tile levels consumed without branches on slot data.  On entry cx holds the first slice's (row,col) bytes (and FP64
+// payload) from chunk_load_rc; the products end in L.acc.
+template <int MODE, int DEPTH, bool FLAT>
+__device__ __forceinline__ void window_slices(const TileView &A, WaveLds &L, int Z0, int nch,
+                                              const double *__restrict__ x,
+                                              const float *__restrict__ x32, bool all64,
+                                              bool x32only, int lane, Chunk &cx, Chunk &cy) {
+  const int nsl = (nch + 63) >> 6;
+  if (DEPTH == 3 && MODE == kAll64) {
+    // Three-deep (FP64 payload only): slice m+2's loads go out after
+    // slice m's gathers, so waiting for those gathers never waits for
+    // them.
+    Chunk cz;
+    if (nsl > 1) chunk_load64_fixed(A, Z0, lane + 64, nch, cy);
+    for (int m = 0; m < nsl; m += 3) {
+      const int c0 = 64 * m + lane;
+      chunk_locate(L, c0, nch, true, cx);
+      chunk_gather64_fixed(L, x, cx);
+      if (m + 2 < nsl) chunk_load64_fixed(A, Z0, c0 + 128, nch, cz);
+      chunk_consume(L, cx, true);
+      if (m + 1 >= nsl) break;
+      chunk_locate(L, c0 + 64, nch, true, cy);
+      chunk_gather64_fixed(L, x, cy);
+      if (m + 3 < nsl) chunk_load64_fixed(A, Z0, c0 + 192, nch, cx);
+      chunk_consume(L, cy, true);
+      if (m + 2 >= nsl) break;
+      chunk_locate(L, c0 + 128, nch, true, cz);
+      chunk_gather64_fixed(L, x, cz);
+      if (m + 4 < nsl) chunk_load64_fixed(A, Z0, c0 + 256, nch, cy);
+      chunk_consume(L, cz, true);
+    }
+  } else if (DEPTH == 3 && MODE != kAll64) {
+    // Three-deep, per-tile levels, fixed-width loads: slice m+2 is
+    // located (LDS) and its loads issued after slice m's gathers.
+    Chunk cz;
+    chunk_locate(L, lane, nch, all64, cx);
+    chunk_load_pay_fixed(A, Z0, lane, nch, cx);
+    if (nsl > 1) {
+      chunk_locate(L, lane + 64, nch, all64, cy);
+      chunk_load_lv_fixed(A, Z0, lane + 64, nch, cy);
+    }
+    for (int m = 0; m < nsl; m += 3) {
+      const int c0 = 64 * m + lane;
+      chunk_gather_lv_fixed(L, x, x32, cx, x32only);
+      if (m + 2 < nsl) {
+        chunk_locate(L, c0 + 128, nch, all64, cz);
+        chunk_load_lv_fixed(A, Z0, c0 + 128, nch, cz);
+      }
+      if (FLAT) chunk_consume_lv_flat(L, cx, x32only); else chunk_consume_lv_fixed(L, cx, x32only);
+      if (m + 1 >= nsl) break;
+      chunk_gather_lv_fixed(L, x, x32, cy, x32only);
+      if (m + 3 < nsl) {
+        chunk_locate(L, c0 + 192, nch, all64, cx);
+        chunk_load_lv_fixed(A, Z0, c0 + 192, nch, cx);
+      }
+      if (FLAT) chunk_consume_lv_flat(L, cy, x32only); else chunk_consume_lv_fixed(L, cy, x32only);
+      if (m + 2 >= nsl) break;
+      chunk_gather_lv_fixed(L, x, x32, cz, x32only);
+      if (m + 4 < nsl) {
+        chunk_locate(L, c0 + 256, nch, all64, cy);
+        chunk_load_lv_fixed(A, Z0, c0 + 256, nch, cy);
+      }
+      if (FLAT) chunk_consume_lv_flat(L, cz, x32only); else chunk_consume_lv_fixed(L, cz, x32only);
+    }
+  } else if (all64) {
+    // Two-deep: slice m+1's loads need no metadata and go out right after
+    // slice m is located.
+    for (int m = 0; m < nsl; m += 2) {
+      const int c0 = 64 * m + lane;
+      chunk_locate(L, c0, nch, true, cx);
+      if (m + 1 < nsl) chunk_load_rc(A, Z0, c0 + 64, nch, true, cy);
+      chunk_gather(L, x, x32, true, cx);
+      chunk_consume(L, cx, true);
+      if (m + 1 >= nsl) break;
+      chunk_locate(L, c0 + 64, nch, true, cy);
+      if (m + 2 < nsl) chunk_load_rc(A, Z0, c0 + 128, nch, true, cx);
+      chunk_gather(L, x, x32, true, cy);
+      chunk_consume(L, cy, true);
+    }
+  } else {
+    // Two-deep, per-tile levels: slice m+1 is located and its loads issued
+    // before slice m's gathers.
+    chunk_locate(L, lane, nch, false, cx);
+    chunk_load_payload(A, Z0, lane, cx);
+    for (int m = 0; m < nsl; m += 2) {
+      const int c1 = 64 * (m + 1) + lane;
+      if (m + 1 < nsl) {
+        chunk_load_rc(A, Z0, c1, nch, false, cy);
+        chunk_locate(L, c1, nch, false, cy);
+        chunk_load_payload(A, Z0, c1, cy);
+      }
+      chunk_gather(L, x, x32, false, cx);
+      chunk_consume(L, cx, false);
+      if (m + 1 >= nsl) break;
+      if (m + 2 < nsl) {
+        chunk_load_rc(A, Z0, c1 + 64, nch, false, cx);
+        chunk_locate(L, c1 + 64, nch, false, cx);
+        chunk_load_payload(A, Z0, c1 + 64, cx);
+      }
+      chunk_gather(L, x, x32, false, cy);
+      chunk_consume(L, cy, false);
+    }
+  }
+}
+
+// y rows of tile-rows [desc.tr0, desc.tr1) through epi(row, value).
+//
+// A descriptor whose tiles fit one window (kDescOneWindow, set by the host
+// partition: at most kWinRows tile-rows, kWinTiles tiles and 4 * kWinChunks
+// slots) is staged with every load it needs in flight at once: each tile's
+// offset, column, tile-row, entry count, level and both scales, and the first
+// slice's (row,col) bytes.  Other descriptors walk their tile-rows in windows
+// that the limits cut, a tile-row's partial sum carried across a cut.
+// ONEWIN = false compiles the walk only, with the branching per-level
+// consume: callers whose descriptors never carry the flag (the solvers, whose
+// loop state shares the register budget) keep those registers free.
+template <int MODE, bool COUNT, int DEPTH = 3, bool ONEWIN = true, typename Epi>
 __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &desc,
                                           const double *__restrict__ x,
                                           const float *__restrict__ x32,
@@ -422,6 +614,59 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
   for (int i = lane; i < kWinRows * 16; i += 64) L.acc[i] = 0.0;
   int ta = __builtin_amdgcn_readfirstlane(desc.ta);
   int Za = __builtin_amdgcn_readfirstlane(desc.Z0);
+  const int ja = lane, jb = lane + 64;
+  if (ONEWIN && (__builtin_amdgcn_readfirstlane(desc.flags) & kDescOneWindow)) {
+    const int nt = T1 - ta;
+    const int Z0 = Za, Z1 = __builtin_amdgcn_readfirstlane(desc.Z1);
+    const int nch = (Z1 - Z0) >> 2;
+    int oa = 0, ob = 0, ca = 0, cb = 0, wa = 0, wb = 0, na = 0, nb = 0, la = 0, lb = 0;
+    float ha = 0.0f, hb = 0.0f, qa = 0.0f, qb = 0.0f;
+    if (ja < nt) {
+      oa = A.tile_off[ta + ja];
+      ca = A.tile_col[ta + ja];
+      wa = A.tile_row[ta + ja];
+      na = A.tile_nz[ta + ja];
+      if (MODE != kAll64) {
+        la = tile_level<MODE>(A, seglev, ta + ja, force64);
+        ha = A.s16[ta + ja];
+        qa = A.s8[ta + ja];
+      }
+    }
+    if (jb < nt) {
+      ob = A.tile_off[ta + jb];
+      cb = A.tile_col[ta + jb];
+      wb = A.tile_row[ta + jb];
+      nb = A.tile_nz[ta + jb];
+      if (MODE != kAll64) {
+        lb = tile_level<MODE>(A, seglev, ta + jb, force64);
+        hb = A.s16[ta + jb];
+        qb = A.s8[ta + jb];
+      }
+    }
+    Chunk cx, cy;
+    chunk_load_rc(A, Z0, lane, nch, all64, cx);
+    // Tile j's payload ends where tile j + 1's begins (lane + 1's offset;
+    // for lane 63's first tile, lane 0's second), the last tile's at Z1.
+    const int oa_next = __shfl(oa, (lane + 1) & 63);
+    const int ob_next = __shfl(ob, (lane + 1) & 63);
+    const int ob_first = __shfl(ob, 0);
+    if (ja < nt) {
+      const int e = ja + 1 < nt ? (lane < 63 ? oa_next : ob_first) : Z1;
+      stage_tile<MODE, COUNT>(L, ja, oa - Z0, e - Z0, ca, wa - tr0, na, la,
+                              la == MF_PREC_FP16 ? ha : qa, cnt);
+    }
+    if (jb < nt) {
+      const int e = jb + 1 < nt ? ob_next : Z1;
+      stage_tile<MODE, COUNT>(L, jb, ob - Z0, e - Z0, cb, wb - tr0, nb, lb,
+                              lb == MF_PREC_FP16 ? hb : qb, cnt);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (nch > 0) window_slices<MODE, DEPTH, true>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane; i < (tr1 - tr0) * 16; i += 64) epi(tr0 * 16 + i, L.acc[i]);
+    __builtin_amdgcn_wave_barrier();
+    return;
+  }
   int ra = tr0;
   while (ra < tr1) {
     const int rlim = ra + kWinRows < tr1 ? ra + kWinRows : tr1;
@@ -436,7 +681,6 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
       const int Z0 = Za;
       // Window metadata into registers: payload offsets of tiles lane,
       // lane + 64 and 128 (one past the last tile when the window is full).
-      const int ja = lane, jb = lane + 64;
       int oa = 0, ob = 0, oc = 0;
       if (ja <= nt) oa = A.tile_off[ta + ja];
       if (jb <= nt) ob = A.tile_off[ta + jb];
@@ -478,138 +722,20 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
       // The first slice's loads that need no metadata go out now.
       Chunk cx, cy;
       chunk_load_rc(A, Z0, lane, nch, all64, cx);
-      // Metadata into LDS; every chunk of tile j records j.  The payload
-      // end of tile j is the offset of tile j + 1 (shuffled in uniform
-      // control flow: lane + 1's oa, for lane 63 lane 0's ob; lane + 1's ob,
-      // for lane 63 oc).
+      // Metadata into LDS.  The payload end of tile j is the offset of tile
+      // j + 1 (shuffled in uniform control flow: lane + 1's oa, for lane 63
+      // lane 0's ob; lane + 1's ob, for lane 63 oc).
       const int na_next = __shfl(oa, (lane + 1) & 63);
       const int nb_next = __shfl(ob, (lane + 1) & 63);
       const int ob_first = __shfl(ob, 0);
-      if (ja < nt) {
-        const int o = oa - Z0;
-        const int e = (lane < 63 ? na_next : ob_first) - Z0;
-        L.tcol[ja] = ca << 4;
-        L.trow[ja] = (uint8_t)(wa - ra);
-        L.tend[ja] = o + na + 1;
-        for (int q = o >> 2; q < (e >> 2); ++q) L.ctile[q] = (uint8_t)ja;
-        if (MODE != kAll64) {
-          L.tprec[ja] = (uint8_t)la;
-          L.tscale[ja] = sa;
-          if (COUNT) cnt->add(la);
-        }
-      }
-      if (jb < nt) {
-        const int o = ob - Z0;
-        const int e = (lane < 63 ? nb_next : oc) - Z0;
-        L.tcol[jb] = cb << 4;
-        L.trow[jb] = (uint8_t)(wb - ra);
-        L.tend[jb] = o + nb + 1;
-        for (int q = o >> 2; q < (e >> 2); ++q) L.ctile[q] = (uint8_t)jb;
-        if (MODE != kAll64) {
-          L.tprec[jb] = (uint8_t)lb;
-          L.tscale[jb] = sb;
-          if (COUNT) cnt->add(lb);
-        }
-      }
+      if (ja < nt)
+        stage_tile<MODE, COUNT>(L, ja, oa - Z0, (lane < 63 ? na_next : ob_first) - Z0, ca,
+                                wa - ra, na, la, sa, cnt);
+      if (jb < nt)
+        stage_tile<MODE, COUNT>(L, jb, ob - Z0, (lane < 63 ? nb_next : oc) - Z0, cb, wb - ra, nb,
+                                lb, sb, cnt);
       __builtin_amdgcn_wave_barrier();
-      // Slices of 64 chunks, with the loads of slice m+1 in flight while
-      // slice m is gathered and consumed.  FP64 payload: slice m+1's loads
-      // need no metadata and go out right after slice m is located.  Per-tile
-      // levels: slice m+1 is located and its loads issued before slice m's
-      // gathers.  Two chunk buffers in fixed roles, so no register copy
-      // waits for loads in flight.
-      const int nsl = (nch + 63) >> 6;
-      if (DEPTH == 3 && MODE == kAll64) {
-        // Three-deep (FP64 payload only): slice m+2's loads go out after
-        // slice m's gathers, so waiting for those gathers never waits for
-        // them.
-        Chunk cz;
-        if (nsl > 1) chunk_load64_fixed(A, Z0, lane + 64, nch, cy);
-        for (int m = 0; m < nsl; m += 3) {
-          const int c0 = 64 * m + lane;
-          chunk_locate(L, c0, nch, true, cx);
-          chunk_gather64_fixed(L, x, cx);
-          if (m + 2 < nsl) chunk_load64_fixed(A, Z0, c0 + 128, nch, cz);
-          chunk_consume(L, cx, true);
-          if (m + 1 >= nsl) break;
-          chunk_locate(L, c0 + 64, nch, true, cy);
-          chunk_gather64_fixed(L, x, cy);
-          if (m + 3 < nsl) chunk_load64_fixed(A, Z0, c0 + 192, nch, cx);
-          chunk_consume(L, cy, true);
-          if (m + 2 >= nsl) break;
-          chunk_locate(L, c0 + 128, nch, true, cz);
-          chunk_gather64_fixed(L, x, cz);
-          if (m + 4 < nsl) chunk_load64_fixed(A, Z0, c0 + 256, nch, cy);
-          chunk_consume(L, cz, true);
-        }
-      } else if (DEPTH == 3 && MODE != kAll64) {
-        // Three-deep, per-tile levels, fixed-width loads: slice m+2 is
-        // located (LDS) and its loads issued after slice m's gathers.
-        Chunk cz;
-        chunk_locate(L, lane, nch, all64, cx);
-        chunk_load_pay_fixed(A, Z0, lane, nch, cx);
-        if (nsl > 1) {
-          chunk_locate(L, lane + 64, nch, all64, cy);
-          chunk_load_lv_fixed(A, Z0, lane + 64, nch, cy);
-        }
-        for (int m = 0; m < nsl; m += 3) {
-          const int c0 = 64 * m + lane;
-          chunk_gather_lv_fixed(L, x, x32, cx, x32only);
-          if (m + 2 < nsl) {
-            chunk_locate(L, c0 + 128, nch, all64, cz);
-            chunk_load_lv_fixed(A, Z0, c0 + 128, nch, cz);
-          }
-          chunk_consume_lv_fixed(L, cx, x32only);
-          if (m + 1 >= nsl) break;
-          chunk_gather_lv_fixed(L, x, x32, cy, x32only);
-          if (m + 3 < nsl) {
-            chunk_locate(L, c0 + 192, nch, all64, cx);
-            chunk_load_lv_fixed(A, Z0, c0 + 192, nch, cx);
-          }
-          chunk_consume_lv_fixed(L, cy, x32only);
-          if (m + 2 >= nsl) break;
-          chunk_gather_lv_fixed(L, x, x32, cz, x32only);
-          if (m + 4 < nsl) {
-            chunk_locate(L, c0 + 256, nch, all64, cy);
-            chunk_load_lv_fixed(A, Z0, c0 + 256, nch, cy);
-          }
-          chunk_consume_lv_fixed(L, cz, x32only);
-        }
-      } else if (all64) {
-        for (int m = 0; m < nsl; m += 2) {
-          const int c0 = 64 * m + lane;
-          chunk_locate(L, c0, nch, true, cx);
-          if (m + 1 < nsl) chunk_load_rc(A, Z0, c0 + 64, nch, true, cy);
-          chunk_gather(L, x, x32, true, cx);
-          chunk_consume(L, cx, true);
-          if (m + 1 >= nsl) break;
-          chunk_locate(L, c0 + 64, nch, true, cy);
-          if (m + 2 < nsl) chunk_load_rc(A, Z0, c0 + 128, nch, true, cx);
-          chunk_gather(L, x, x32, true, cy);
-          chunk_consume(L, cy, true);
-        }
-      } else {
-        chunk_locate(L, lane, nch, false, cx);
-        chunk_load_payload(A, Z0, lane, cx);
-        for (int m = 0; m < nsl; m += 2) {
-          const int c1 = 64 * (m + 1) + lane;
-          if (m + 1 < nsl) {
-            chunk_load_rc(A, Z0, c1, nch, false, cy);
-            chunk_locate(L, c1, nch, false, cy);
-            chunk_load_payload(A, Z0, c1, cy);
-          }
-          chunk_gather(L, x, x32, false, cx);
-          chunk_consume(L, cx, false);
-          if (m + 1 >= nsl) break;
-          if (m + 2 < nsl) {
-            chunk_load_rc(A, Z0, c1 + 64, nch, false, cx);
-            chunk_locate(L, c1 + 64, nch, false, cx);
-            chunk_load_payload(A, Z0, c1 + 64, cx);
-          }
-          chunk_gather(L, x, x32, false, cy);
-          chunk_consume(L, cy, false);
-        }
-      }
+      window_slices<MODE, DEPTH, ONEWIN>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
       __builtin_amdgcn_wave_barrier();
     }
     // Tile-rows [ra, R) are complete; tile-row R has started iff tile_ptr[R] < tb.
@@ -643,8 +769,8 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, int tr0, int tr1,
                                           const uint8_t *__restrict__ seglev, int force64,
                                           WaveLds &L, int lane, PrecCount *cnt, Epi epi) {
   if (tr0 >= tr1) return;
-  wave_spmv<MODE, COUNT, DEPTH>(A, wave_desc(A, tr0, tr1), x, x32, seglev, force64, L, lane, cnt,
-                                epi);
+  wave_spmv<MODE, COUNT, DEPTH, false>(A, wave_desc(A, tr0, tr1), x, x32, seglev, force64, L, lane,
+                                       cnt, epi);
 }
 
 // ---------------------------------------------------------------------------
@@ -877,6 +1003,15 @@ __device__ __forceinline__ void grid_sync(GridSync *gs, unsigned &epoch, int *ld
 // always separate two uses of one slot).
 constexpr int kMaxGridSum = 4;
 
+// A wave-uniform double moved to scalar registers, so that loop state which
+// every lane holds alike does not occupy vector registers.
+__device__ __forceinline__ double uniform_double(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 template <int NV>
 __device__ __forceinline__ void grid_sum(double (&v)[NV], double *partials, GridSync *gs,
                                          unsigned &epoch, int &slot, double *lds, int *lds_i) {
@@ -905,7 +1040,7 @@ __device__ __forceinline__ void grid_sum(double (&v)[NV], double *partials, Grid
     }
   });
 #pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = res[i];
+  for (int i = 0; i < NV; ++i) v[i] = uniform_double(res[i]);
   slot ^= 1;
 }
 

@@ -179,31 +179,38 @@ std::vector<int> balance_tilerows(const mf_tiled *t, int nwaves) {
   return b;
 }
 
-std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz) {
-  std::vector<WaveDesc> out;
-  const int tm = t->tilem;
+void window_partition_range(const mf_tiled *t, int r0, int r1, int target_nnz,
+                            std::vector<WaveDesc> &out) {
   auto nnz_of = [&](int r) { return t->pay_off[t->tile_ptr[r + 1]] - t->pay_off[t->tile_ptr[r]]; };
-  int r = 0;
-  while (r < tm) {
-    const int r0 = r;
+  int r = r0;
+  while (r < r1) {
+    const int rb = r;
     int nz = 0;
     // first tile-row always; then extend while the window limits and the
     // entry target hold
     do {
       nz += nnz_of(r);
       ++r;
-    } while (r < tm && r - r0 < kWinRows && t->tile_ptr[r + 1] - t->tile_ptr[r0] <= kWinTiles &&
+    } while (r < r1 && r - rb < kWinRows && t->tile_ptr[r + 1] - t->tile_ptr[rb] <= kWinTiles &&
              nz + nnz_of(r) <= target_nnz && nz + nnz_of(r) <= 4 * kWinChunks);
     WaveDesc d;
-    d.tr0 = r0;
+    d.tr0 = rb;
     d.tr1 = r;
-    d.ta = t->tile_ptr[r0];
+    d.ta = t->tile_ptr[rb];
     d.T1 = t->tile_ptr[r];
     d.Z0 = t->pay_off[d.ta];
     d.Z1 = t->pay_off[d.T1];
-    d.pad0 = d.pad1 = 0;
+    const bool one = d.tr1 - d.tr0 <= kWinRows && d.T1 - d.ta <= kWinTiles &&
+                     d.Z1 - d.Z0 <= 4 * kWinChunks;
+    d.flags = one ? kDescOneWindow : 0;
+    d.pad1 = 0;
     out.push_back(d);
   }
+}
+
+std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz) {
+  std::vector<WaveDesc> out;
+  window_partition_range(t, 0, t->tilem, target_nnz, out);
   return out;
 }
 

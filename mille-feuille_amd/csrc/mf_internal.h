@@ -21,12 +21,16 @@ constexpr int kWinTiles = 128;
 constexpr int kWinRows = 16;
 constexpr int kWinChunks = 1024;
 
-// Work of one wave of the stand-alone streamed SpMV: tile-rows [tr0, tr1),
-// their tiles [ta, T1) and payload entries [Z0, Z1), precomputed so that a
-// wave starts loading payload without first walking the tile pointers.
+// Work of one wave of the streamed SpMV: tile-rows [tr0, tr1), their tiles
+// [ta, T1) and payload entries [Z0, Z1), precomputed so that a wave starts
+// loading payload without first walking the tile pointers.  flags bit 0
+// (kDescOneWindow): the tiles fit one window (at most kWinRows tile-rows,
+// kWinTiles tiles, 4 * kWinChunks slots).
 struct WaveDesc {
-  int tr0, tr1, ta, T1, Z0, Z1, pad0, pad1;
+  int tr0, tr1, ta, T1, Z0, Z1, flags, pad1;
 };
+constexpr int kDescOneWindow = 1;
+
 
 // Thread-local error message (mf_last_error).
 void set_error(const std::string &msg);
@@ -110,6 +114,11 @@ std::vector<int> balance_tilerows(const mf_tiled *t, int nwaves);
 // chunks) unless a single tile-row is larger than a window (such a tile-row
 // gets a wave of its own).  Needs pay_off.
 std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz);
+// Tile-rows [r0, r1) cut into descriptors of at most target_nnz payload
+// slots, each within one window's limits unless a single tile-row exceeds
+// them (such a descriptor walks several windows).
+void window_partition_range(const mf_tiled *t, int r0, int r1, int target_nnz,
+                            std::vector<WaveDesc> &out);
 // Default stand-alone SpMV kernel (MF_SPMV_STREAM or MF_SPMV_ROWGROUP).
 int default_spmv_kernel(const mf_tiled *t);
 // No tile-row holds more than 4x the mean tile count + 8 (the row-group

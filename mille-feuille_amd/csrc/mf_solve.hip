@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
   double rr = red[0];
   const double bb = red[1];
-  const double stop2 = a.tol2 * bb;
+  const double stop2 = uniform_double(a.tol2 * bb);
   double rr_ref = rr, gamma_old = 1.0, alpha_old = 1.0;
   bool first = true;
   int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0;
@@ -336,6 +336,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       break;
     }
     first = false;
+    alpha = uniform_double(alpha);
+    beta = uniform_double(beta);
     gamma_old = gamma;
     alpha_old = alpha;
     ++it;
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
   double rho = red[0], rr = red[0];
   const double bb = red[1];
-  const double stop2 = a.tol2 * bb;
+  const double stop2 = uniform_double(a.tol2 * bb);
   double rr_ref = rr;
   int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0, restarts = 0;
   bool restart = false;
@@ -483,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
       restart = true;
       continue;
     }
-    const double alpha = rho / vr[0];
+    const double alpha = uniform_double(rho / vr[0]);
     // s = r - alpha v (rule policy: |s_new - s_old| per segment)
     const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
     for (int base = row0; base < row1; base += 64) {
@@ -511,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
         tw[1] += tvv * tvv;
       });
     grid_sum<2>(tw, a.partials, a.gs, epoch, slot, lds, &lds_i);
-    const double omega = tw[1] != 0.0 && isfinite(tw[1]) ? tw[0] / tw[1] : 0.0;
+    const double omega = uniform_double(tw[1] != 0.0 && isfinite(tw[1]) ? tw[0] / tw[1] : 0.0);
     // x += alpha p + omega s, r = s - omega t, rho' = r.rh, ||r||^2
     double nw[2] = {0.0, 0.0};
     for (int i = row0 + lane; i < row1; i += 64) {
@@ -556,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
       if (omega == 0.0 || rho == 0.0 || rho_new == 0.0 || !isfinite(rho_new)) restart = true;
       if (restart) continue;
     }
-    const double beta = (rho_new / rho) * (alpha / omega);
+    const double beta = uniform_double((rho_new / rho) * (alpha / omega));
     rho = rho_new;
     rr = rr_new;
     // p = r + beta (p - omega v)
@@ -688,7 +690,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_res(SolveArgs a) {
   grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
   double rr = red[0];
   const double bb = red[1];
-  const double stop2 = a.tol2 * bb;
+  const double stop2 = uniform_double(a.tol2 * bb);
   double rr_ref = rr, gamma_old = 1.0, alpha_old = 1.0;
   bool first = true;
   int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0;
@@ -803,6 +805,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_res(SolveArgs a) {
       break;
     }
     first = false;
+    alpha = uniform_double(alpha);
+    beta = uniform_double(beta);
     gamma_old = gamma;
     alpha_old = alpha;
     ++it;

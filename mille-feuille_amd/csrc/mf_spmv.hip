@@ -13,8 +13,13 @@
 
 namespace mf {
 
+// Occupancy: the latency of a wave's window (metadata -> payload -> x
+// gathers) is hidden by waves in flight, so the register budget is capped at
+// 64 VGPRs (8 waves per SIMD; 32 waves x 4.75 KiB of LDS per CU) for FP64
+// payloads and 80 (6 waves) for per-tile levels, the most that compile
+// without spilling.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_spmv(TileView A, const WaveDesc *__restrict__ desc,
+__global__ __launch_bounds__(256, MODE == kAll64 ? 8 : 6) void k_spmv(TileView A, const WaveDesc *__restrict__ desc,
                                               int nwaves, const double *__restrict__ x,
                                               const float *__restrict__ x32,
                                               double *__restrict__ y) {
