@@ -595,10 +595,12 @@ __device__ __forceinline__ void window_slices(const TileView &A, WaveLds &L, int
 // offset, column, tile-row, entry count, level and both scales, and the first
 // slice's (row,col) bytes.  Other descriptors walk their tile-rows in windows
 // that the limits cut, a tile-row's partial sum carried across a cut.
-// ONEWIN = false compiles the walk only, with the branching per-level
-// consume: callers whose descriptors never carry the flag (the solvers, whose
-// loop state shares the register budget) keep those registers free.
-template <int MODE, bool COUNT, int DEPTH = 3, bool ONEWIN = true, typename Epi>
+// ONEWIN = false compiles the walk only, for callers whose descriptors never
+// carry the flag.  FLAT selects the branch-free per-level consume; kernels
+// whose loop state shares the register budget (the solvers) use the
+// branching one, which keeps fewer values live.
+template <int MODE, bool COUNT, int DEPTH = 3, bool ONEWIN = true, bool FLAT = ONEWIN,
+          typename Epi>
 __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &desc,
                                           const double *__restrict__ x,
                                           const float *__restrict__ x32,
@@ -661,7 +663,7 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
                               lb == MF_PREC_FP16 ? hb : qb, cnt);
     }
     __builtin_amdgcn_wave_barrier();
-    if (nch > 0) window_slices<MODE, DEPTH, true>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
+    if (nch > 0) window_slices<MODE, DEPTH, FLAT>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
     __builtin_amdgcn_wave_barrier();
     for (int i = lane; i < (tr1 - tr0) * 16; i += 64) epi(tr0 * 16 + i, L.acc[i]);
     __builtin_amdgcn_wave_barrier();
@@ -735,7 +737,7 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
         stage_tile<MODE, COUNT>(L, jb, ob - Z0, (lane < 63 ? nb_next : oc) - Z0, cb, wb - ra, nb,
                                 lb, sb, cnt);
       __builtin_amdgcn_wave_barrier();
-      window_slices<MODE, DEPTH, ONEWIN>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
+      window_slices<MODE, DEPTH, FLAT>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
       __builtin_amdgcn_wave_barrier();
     }
     // Tile-rows [ra, R) are complete; tile-row R has started iff tile_ptr[R] < tb.
