@@ -346,41 +346,79 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
     // the segment norms of r_new for its next inference.
     const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
     const bool lupd = learned && !force64 && (it % a.check_interval) == 0;
-    for (int base = row0; base < row1; base += 64) {
-      const int i = base + lane;
-      double d = 0.0;
-      if (i < row1) {
-        double rv, rn;
-        if (lowv) {
-          rv = a.r32[i];
-          const double pv = rv + beta * (double)a.p32[i];
-          const double sv = (double)a.q32[i] + beta * (double)a.s32[i];
-          const float pf = (float)pv, sf = (float)sv;
-          a.p32[i] = pf;
-          a.s32[i] = sf;
-          a.x[i] += alpha * (double)pf;
-          const float rf = (float)(rv - alpha * (double)sf);
-          a.r32[i] = rf;
-          rn = rf;
-        } else {
-          rv = a.r[i];
-          const double pv = rv + beta * a.p[i];
-          const double sv = a.q[i] + beta * a.s[i];
-          a.p[i] = pv;
-          a.s[i] = sv;
-          a.x[i] += alpha * pv;
-          rn = rv - alpha * sv;
-          a.r[i] = rn;
+    // Four 64-row groups per step, every load of a step issued before its
+    // arithmetic, so a wave keeps 20 loads in flight instead of 5.
+    constexpr int kU = 4;
+    for (int base = row0; base < row1; base += 64 * kU) {
+      double rv[kU], rn[kU];
+      if (lowv) {
+        float r_[kU], p_[kU], q_[kU], s_[kU];
+        double x_[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int i = base + 64 * u + lane;
+          const bool ok = i < row1;
+          r_[u] = ok ? a.r32[i] : 0.0f;
+          p_[u] = ok ? a.p32[i] : 0.0f;
+          q_[u] = ok ? a.q32[i] : 0.0f;
+          s_[u] = ok ? a.s32[i] : 0.0f;
+          x_[u] = ok ? a.x[i] : 0.0;
         }
-        d = lupd ? rn * rn : fabs(rn - rv);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int i = base + 64 * u + lane;
+          rv[u] = r_[u];
+          const double pv = rv[u] + beta * (double)p_[u];
+          const double sv = (double)q_[u] + beta * (double)s_[u];
+          const float pf = (float)pv, sf = (float)sv;
+          const float rf = (float)(rv[u] - alpha * (double)sf);
+          rn[u] = rf;
+          if (i < row1) {
+            a.p32[i] = pf;
+            a.s32[i] = sf;
+            a.x[i] = x_[u] + alpha * (double)pf;
+            a.r32[i] = rf;
+          }
+        }
+      } else {
+        double r_[kU], p_[kU], q_[kU], s_[kU], x_[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int i = base + 64 * u + lane;
+          const bool ok = i < row1;
+          r_[u] = ok ? a.r[i] : 0.0;
+          p_[u] = ok ? a.p[i] : 0.0;
+          q_[u] = ok ? a.q[i] : 0.0;
+          s_[u] = ok ? a.s[i] : 0.0;
+          x_[u] = ok ? a.x[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int i = base + 64 * u + lane;
+          rv[u] = r_[u];
+          const double pv = rv[u] + beta * p_[u];
+          const double sv = q_[u] + beta * s_[u];
+          rn[u] = rv[u] - alpha * sv;
+          if (i < row1) {
+            a.p[i] = pv;
+            a.s[i] = sv;
+            a.x[i] = x_[u] + alpha * pv;
+            a.r[i] = rn[u];
+          }
+        }
       }
-      if (upd) {
-        d = group16_max(d);
-        if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
-      }
-      if (lupd) {
-        d = group16_sum(d);
-        if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int i = base + 64 * u + lane;
+        double d = i < row1 ? (lupd ? rn[u] * rn[u] : fabs(rn[u] - rv[u])) : 0.0;
+        if (upd) {
+          d = group16_max(d);
+          if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
+        }
+        if (lupd) {
+          d = group16_sum(d);
+          if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d;
+        }
       }
     }
     infer = lupd;

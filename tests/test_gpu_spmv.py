@@ -181,3 +181,58 @@ def test_spmv_empty_and_rectangular(mf, oracle):
     yr = oracle.csr_spmv(rp, cols, va, x)
     np.testing.assert_allclose(y[:m], yr, rtol=0, atol=1e-13)
     assert y[5] == 0 and not y[m:].any()
+
+
+def _arrow(n=3000, dense_rows=16, seed=9):
+    """Arrow matrix: the first `dense_rows` rows are dense (one tile-row of
+    ceil(n/16) tiles and dense_rows*n entries, more than one SpMV window holds:
+    128 tiles, 4096 slots), the rest is a tridiagonal band (tile-rows that fit
+    one window).  Symmetric pattern, random values."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(dense_rows):
+        rows.append(np.full(n, i))
+        cols.append(np.arange(n))
+    for i in range(dense_rows, n):
+        js = [j for j in (i - 1, i, i + 1) if dense_rows <= j < n]
+        js += list(range(dense_rows))  # the arrow's column part
+        rows.append(np.full(len(js), i))
+        cols.append(np.array(js))
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    order = np.lexsort((c, r))
+    r, c = r[order], c[order]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    rp = np.cumsum(rp)
+    return rp, c.astype(np.int32), rng.uniform(-1, 1, len(c))
+
+
+def test_spmv_window_walk_and_one_window_descriptors(mf, oracle):
+    """A tile-row larger than one window (188 tiles, 48000 entries) is walked
+    in several windows with its partial row sums carried; the band's
+    tile-rows take the one-window staging.  Both against the oracle, FP64
+    tiles and per-tile levels, at several wave schedules."""
+    rp, ci, va = _arrow()
+    n = len(rp) - 1
+    tm = _mk(mf, rp, ci, va)
+    a = tm.arrays()
+    tiles_per_row = np.diff(a["tile_ptr"])
+    assert tiles_per_row[0] > 128 and a["csr_offset"][tiles_per_row[0]] > 4096
+    x = np.random.default_rng(3).uniform(-1, 1, n)
+    dx, dy = tm.vector(x), tm.vector()
+    dx32 = tm.vector(x.astype(np.float32), dtype=np.float32)
+    ot = oracle.tile_create(n, n, rp, ci, va)
+    yr = oracle.blockspmv_tiled(ot, x)
+    bound = oracle.csr_spmv(rp, ci, np.abs(va), np.abs(x))
+    levels = np.random.default_rng(4).integers(0, 5, tm.tilenum).astype(np.uint8)
+    ym, blow, bhigh = _dequant_reference(tm, levels, x)
+    for wn in (256, 1024, 4096):
+        print(f"wave_nnz {wn}", flush=True)
+        tm.set_schedule(wn)
+        tm.set_precision(None, uniform=0)
+        tm.spmv(dx, dy, mode=0)
+        assert np.all(np.abs(dy.get()[:n] - yr) <= 1e-14 * bound + 1e-300)
+        tm.set_precision(levels)
+        tm.spmv(dx, dy, mode=1, d_x32=dx32)
+        assert np.all(np.abs(dy.get()[:n] - ym[:n]) <= 2e-6 * blow[:n] + 1e-14 * bhigh[:n] + 1e-300)
