@@ -18,6 +18,20 @@ timed on this host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--policy learned|errbound|rule|fp64]
 
+The other workloads the library is measured on print one JSON line each
+(`config.workload` names them); none replaces the line above:
+
+  --workload convdiff100   BiCGSTAB, 3D 7-point upwind convection-diffusion
+                           100^3 (cell Peclet 0.5, velocity (1, 0.5, 0.25)),
+                           b = A x*, x* ~ U[-1,1] seed 3: FP64 vs mixed policies
+  --workload knn512        batched CG: 512 kNN-graph Laplacians (k = 8, shift
+                           1e-2, n ~ U[1k, 64k] seed 4) x 8 right-hand sides in
+                           one hipGraph-captured persistent launch
+  --workload powerlaw4m    CG on a power-law graph Laplacian (Chung-Lu,
+                           exponent 2.5, n = 4M, ~60M nnz, shift 1e-3):
+                           learned vs rule vs FP64
+  --workload powerlaw1m    the same graph model at n = 1M, 15M nnz
+
 The learned policy uses the MLP shipped with the package (policy_mlp.f32,
 trained on the GPU by millefeuille.policy.train_default).
 """
@@ -39,6 +53,134 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _line(metric, value, unit, args, ms, config, **extra):
+    d = {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1,
+         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+         "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+         "dtype": "f64 (scalars, x, residual, accumulation); f32 products of fp32/fp16/int8 tile payloads",
+         "data": "synthetic (seeded generator)", "config": config}
+    d.update(extra)
+    print(json.dumps(d), flush=True)
+
+
+def _timed_solves(tm, solve, policies, args, n, rp, ci, va, b, dx):
+    """Warm-up, then args.steps solves per policy alternated step by step;
+    mean wall ms per solve, the last report and the FP64 relative residual of
+    its x recomputed on the CPU (oracle CSR product)."""
+    import millefeuille as mf
+    import oracle
+    for _ in range(args.warmup):
+        for pol in policies:
+            r = solve(pol)
+            log(f"warm-up {pol}: {r['iterations']} iterations, {r['time_ms']:.1f} ms")
+    mf.synchronize()
+    out = {pol: [] for pol in policies}
+    reps = {}
+    rel = {}
+    for _ in range(args.steps):
+        for pol in policies:
+            t1 = time.perf_counter()
+            reps[pol] = solve(pol)
+            out[pol].append(time.perf_counter() - t1)
+            rel[pol] = oracle.rel_residual(rp, ci, va, b, dx.get()[:n])
+            log(f"{pol}: {reps[pol]['iterations']} iterations, {1e3 * out[pol][-1]:.1f} ms")
+    res = {}
+    for pol in policies:
+        r = reps[pol]
+        res[pol] = {"ms": round(1e3 * sum(out[pol]) / args.steps, 3),
+                    "kernel_ms": round(r["time_ms"], 3), "iterations": r["iterations"],
+                    "refreshes": r["refreshes"], "fell_back": r["fell_back"], "status": r["status"],
+                    "rel_residual_fp64_cpu": rel[pol], "prec_hist": r["prec_hist"]}
+        log(f"{pol}: {res[pol]}")
+    return res
+
+
+def other_workload(args):
+    import millefeuille as mf
+    from millefeuille import gen, policy as polmod
+    t0 = time.time()
+    if args.workload == "convdiff100":
+        rp, ci, va = gen.convdiff3d(100)
+        n = len(rp) - 1
+        xstar = gen.uniform_vector(n, 3)
+        b = gen.csr_matvec(rp, ci, va, xstar)
+        tm = mf.TiledMatrix(rp, ci, va)
+        polmod.install(tm, polmod.load_or_train())
+        log(f"setup: n={n} nnz={rp[-1]} tiles={tm.tilenum} ({time.time() - t0:.1f}s)")
+        db, dx = tm.vector(b), tm.vector()
+
+        def solve(pol):
+            dx.zero()
+            return tm.bicgstab(db, dx, allow_not_converged=True, tol=1e-10, policy=pol,
+                               check_interval=args.check_interval)
+        res = _timed_solves(tm, solve, ["learned", "errbound", "fp64"], args, n, rp, ci, va, b, dx)
+        cfg = {"workload": "BiCGSTAB, 3D 7-point upwind convection-diffusion 100^3, cell Peclet 0.5, "
+                           "velocity (1,0.5,0.25), b = A x*, x* ~ U[-1,1] seed 3, x0 = 0, tol 1e-10",
+               "n": n, "nnz": int(rp[-1]), "tiles": tm.tilenum}
+        _line("time-to-1e-10 residual (FP64-checked): mixed vs FP64 single-kernel BiCGSTAB",
+              res["learned"]["ms"], "ms", args, res["learned"]["ms"], cfg, solves=res,
+              speedup_vs_fp64=round(res["fp64"]["ms"] / res["learned"]["ms"], 3))
+    elif args.workload in ("powerlaw4m", "powerlaw1m"):
+        big = args.workload == "powerlaw4m"
+        rp, ci, va = gen.powerlaw_laplacian(4_000_000 if big else 1_000_000,
+                                            60_000_000 if big else 15_000_000, seed=5)
+        n = len(rp) - 1
+        log(f"generated n={n} nnz={rp[-1]} ({time.time() - t0:.1f}s)")
+        b = gen.uniform_vector(n, 2)
+        tm = mf.TiledMatrix(rp, ci, va)
+        polmod.install(tm, polmod.load_or_train())
+        log(f"setup: tiles={tm.tilenum} ({time.time() - t0:.1f}s)")
+        db, dx = tm.vector(b), tm.vector()
+
+        def solve(pol):
+            dx.zero()
+            return tm.cg(db, dx, allow_not_converged=True, tol=1e-10, policy=pol,
+                         check_interval=args.check_interval)
+        res = _timed_solves(tm, solve, ["learned", "rule", "fp64"], args, n, rp, ci, va, b, dx)
+        cfg = {"workload": "CG, power-law graph Laplacian (Chung-Lu, exponent 2.5, seed 5) + 1e-3 I, "
+                           "b ~ U[-1,1] seed 2, x0 = 0, tol 1e-10",
+               "n": n, "nnz": int(rp[-1]), "tiles": tm.tilenum}
+        _line("time-to-1e-10 residual (FP64-checked): learned vs rule policy, single-kernel CG",
+              res["learned"]["ms"], "ms", args, res["learned"]["ms"], cfg, solves=res,
+              speedup_vs_rule=round(res["rule"]["ms"] / res["learned"]["ms"], 3),
+              speedup_vs_fp64=round(res["fp64"]["ms"] / res["learned"]["ms"], 3))
+    else:  # knn512
+        nsys, nrhs = 512, 8
+        sizes = np.random.default_rng(4).integers(1000, 65537, nsys)
+        tms, bs = [], []
+        for k, nk in enumerate(sizes):
+            rp, ci, va = gen.knn_laplacian(int(nk), 8, seed=4 + k, shift=1e-2)
+            tms.append(mf.TiledMatrix(rp, ci, va))
+            bs.append(np.stack([gen.uniform_vector(int(nk), 1000 * k + j) for j in range(nrhs)]))
+            if (k + 1) % 64 == 0:
+                log(f"built {k + 1} systems ({time.time() - t0:.1f}s)")
+        _, rep = mf.batch_cg(tms, bs, nrhs=nrhs, tol=1e-10, policy="fp64", tune_iters=20)
+        waves = rep["batch_waves"]
+        log(f"batch autotuner: waves per item {waves}, {rep['tune']}")
+        res = {}
+        for pol in ("errbound", "fp64"):
+            for _ in range(args.warmup):
+                mf.batch_cg(tms, bs, nrhs=nrhs, tol=1e-10, policy=pol, batch_waves=waves)
+            tot = []
+            for _ in range(args.steps):
+                _, rep = mf.batch_cg(tms, bs, nrhs=nrhs, tol=1e-10, policy=pol, batch_waves=waves)
+                tot.append(rep["total_ms"])
+            per = np.asarray(rep["per_system_ms"])
+            res[pol] = {"total_ms": round(float(np.mean(tot)), 3),
+                        "per_system_ms_p50": round(float(np.percentile(per, 50)), 3),
+                        "per_system_ms_p99": round(float(np.percentile(per, 99)), 3),
+                        "iterations_mean": round(float(np.mean(rep["iterations"])), 1),
+                        "iterations_max": int(np.max(rep["iterations"])),
+                        "max_rel_residual": float(np.max(rep["rel_residual"]))}
+            log(f"{pol}: {res[pol]}")
+        cfg = {"workload": "batched CG: 512 kNN-graph Laplacians (k=8, shift 1e-2, n ~ U[1k,64k] seed 4) "
+                           "x 8 right-hand sides U[-1,1], one hipGraph-captured persistent launch, tol 1e-10",
+               "systems": nsys, "rhs": nrhs, "sum_n": int(sizes.sum()), "waves_per_item": waves}
+        _line("batched solve time to 1e-10 (every system and right-hand side)",
+              res["errbound"]["total_ms"], "ms", args, res["errbound"]["total_ms"], cfg, solves=res,
+              speedup_vs_fp64=round(res["fp64"]["total_ms"] / res["errbound"]["total_ms"], 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,11 +195,16 @@ def main():
     ap.add_argument("--spmv-runs", type=int, default=50)
     ap.add_argument("--wave-nnz", type=int, default=1024,
                     help="stand-alone SpMV entries per wave (0: pick with the autotuner)")
+    ap.add_argument("--workload", default="lap3d128",
+                    choices=["lap3d128", "convdiff100", "knn512", "powerlaw4m", "powerlaw1m"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this library runs on one GPU; tile-row sharding over "
                                    "several GPUs is not built; run with --gpus 1"}))
         sys.exit(2)
+    if args.workload != "lap3d128":
+        other_workload(args)
+        return
 
     import millefeuille as mf
     from millefeuille import gen
