@@ -459,7 +459,7 @@ __device__ __forceinline__ WaveDesc wave_desc(const TileView &A, int tr0, int tr
   d.T1 = A.tile_ptr[tr1];
   d.Z0 = A.tile_off[d.ta];
   d.Z1 = A.tile_off[d.T1];
-  d.flags = d.pad1 = 0;
+  d.flags = d.aux = 0;
   return d;
 }
 

@@ -203,7 +203,7 @@ void window_partition_range(const mf_tiled *t, int r0, int r1, int target_nnz,
     const bool one = d.tr1 - d.tr0 <= kWinRows && d.T1 - d.ta <= kWinTiles &&
                      d.Z1 - d.Z0 <= 4 * kWinChunks;
     d.flags = one ? kDescOneWindow : 0;
-    d.pad1 = 0;
+    d.aux = 0;
     out.push_back(d);
   }
 }

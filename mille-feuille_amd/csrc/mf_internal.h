@@ -27,9 +27,13 @@ constexpr int kWinChunks = 1024;
 // (kDescOneWindow): the tiles fit one window (at most kWinRows tile-rows,
 // kWinTiles tiles, 4 * kWinChunks slots).
 struct WaveDesc {
-  int tr0, tr1, ta, T1, Z0, Z1, flags, pad1;
+  int tr0, tr1, ta, T1, Z0, Z1, flags, aux;
 };
 constexpr int kDescOneWindow = 1;
+// flags bit 1 (kDescPiece): tiles [ta, T1) are piece flags >> 8 of a heavy
+// tile-row tr0 split over several waves; aux is the heavy tile-row's index
+// (the solvers add the pieces' row sums).
+constexpr int kDescPiece = 2;
 
 
 // Thread-local error message (mf_last_error).
@@ -73,9 +77,23 @@ struct DeviceLayout {
 };
 
 struct Schedule {
-  // Contiguous tile-row ranges, one per persistent wave, balanced on nnz.
+  // Contiguous tile-row ranges, one per persistent wave, balanced on nnz
+  // (the vector phases' row ownership).
   int nwaves = 0;
   int *wave_tr = nullptr;  // device, nwaves+1 tile-row bounds
+  // SpMV work of each wave: win_desc[wave_win[w] .. wave_win[w+1]), the
+  // window descriptors of its tile-rows except heavy ones, then the pieces of
+  // heavy tile-rows assigned to it.
+  int *wave_win = nullptr;       // device, nwaves+1
+  WaveDesc *win_desc = nullptr;  // device
+  // Heavy tile-rows: 16 FP64 partial row sums per piece, and per heavy
+  // tile-row an arrival counter (zero between SpMVs; the last piece resets
+  // it), the piece count and the first piece's slot.
+  int nheavy = 0;
+  double *heavy_part = nullptr;   // device, 16 * pieces
+  unsigned *heavy_cnt = nullptr;  // device, nheavy
+  int *heavy_n = nullptr;         // device, nheavy
+  int *heavy_base = nullptr;      // device, nheavy
 };
 
 }  // namespace mf

@@ -19,6 +19,7 @@
 // fell by `refresh_ratio` since the last recomputation (reliable updates,
 // i.e. iterative refinement inside one Krylov run).
 #include <algorithm>
+#include <queue>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -54,6 +55,12 @@ struct SolveArgs {
   float *r32, *q32;           // mixed CG: r and q in FP32 (p in p32)
   uint8_t *seg_p, *seg_s;     // per-segment levels (rule policy)
   const int *wave_tr;
+  const int *wave_win;       // per wave, bounds of its SpMV descriptors
+  const WaveDesc *win_desc;  // window descriptors and heavy tile-row pieces
+  double *heavy_part;        // 16 partial row sums per heavy tile-row piece
+  unsigned *heavy_cnt;       // pieces of each heavy tile-row done
+  const int *heavy_n;        // pieces of each heavy tile-row
+  const int *heavy_base;     // first piece slot of each heavy tile-row
   double *partials;
   GridSync *gs;
   double *stats;
@@ -128,6 +135,59 @@ __device__ __forceinline__ void flush_hist(const PrecCount &c, unsigned long lon
   }
 }
 
+// SpMV of the calling wave's share: its descriptors (whole tile-rows) with
+// epi(row, value), then its pieces of heavy tile-rows.  Piece k of heavy
+// tile-row h stores its 16 partial row sums in its own slot (agent-scope
+// atomic stores) and counts itself done; the piece that completes the count
+// (the last to finish, so no wave ever waits for another) adds the slots in
+// a fixed order (deterministic), resets the counter for the next SpMV and
+// runs epi for the 16 rows.
+template <int MODE, bool COUNT, typename Epi>
+__device__ __forceinline__ void solver_spmv(const SolveArgs &a, const double *__restrict__ x,
+                                            const float *__restrict__ x32,
+                                            const uint8_t *__restrict__ seglev, int force64,
+                                            WaveLds &L, int lane, PrecCount *cnt, Epi epi) {
+  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int d0 = __builtin_amdgcn_readfirstlane(a.wave_win[wid]);
+  const int d1 = __builtin_amdgcn_readfirstlane(a.wave_win[wid + 1]);
+  for (int d = d0; d < d1; ++d) {
+    const WaveDesc &desc = a.win_desc[d];
+    const int flags = __builtin_amdgcn_readfirstlane(desc.flags);
+    const bool piece = flags & kDescPiece;
+    const int h = piece ? __builtin_amdgcn_readfirstlane(desc.aux) : 0;
+    const int base = piece ? __builtin_amdgcn_readfirstlane(a.heavy_base[h]) : 0;
+    double *slot = a.heavy_part + 16 * (size_t)(base + (flags >> 8));
+    wave_spmv<MODE, COUNT, 3, false>(a.A, desc, x, x32, seglev, force64, L, lane, cnt,
+      [&](const int i, const double v) {
+        if (piece)
+          __hip_atomic_store(slot + (i & 15), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+          epi(i, v);
+      });
+    if (!piece) continue;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    unsigned done = 0;
+    if (lane == 0)
+      done = __hip_atomic_fetch_add(a.heavy_cnt + h, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    done = __builtin_amdgcn_readfirstlane(done);
+    const int np = __builtin_amdgcn_readfirstlane(a.heavy_n[h]);
+    if (done == (unsigned)np) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // lane (q, row): pieces q, q+4, ... of the row, then the four partial
+      // sums in q order
+      const int row = lane & 15, q = lane >> 4;
+      double sum = 0.0;
+      for (int k = q; k < np; k += 4)
+        sum += __hip_atomic_load(a.heavy_part + 16 * (size_t)(base + k) + row, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+      const double s1 = __shfl(sum, row + 16), s2 = __shfl(sum, row + 32), s3 = __shfl(sum, row + 48);
+      const int R = __builtin_amdgcn_readfirstlane(desc.tr0);
+      if (lane < 16) epi(16 * R + lane, ((sum + s1) + s2) + s3);
+      if (lane == 0) __hip_atomic_store(a.heavy_cnt + h, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 // True residual of the wave's tile-rows: r_new = b - A x (FP64), optionally
 // stored in FP64 (store) or FP32 (store32).  Returns the wave's share of
 // ||r_new||^2 (of the stored FP32 values when store32 is given).
@@ -135,7 +195,7 @@ __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int tr0
                                                      int lane, double *store, WaveLds *wl,
                                                      float *store32 = nullptr) {
   double acc = 0.0;
-  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
       double rv = a.b[i] - ax;
       if (store) store[i] = rv;
@@ -156,7 +216,7 @@ __device__ __forceinline__ double restart_rows(const SolveArgs &a, int tr0, int 
                                                bool mixed, bool bicg, WaveLds *wl,
                                                bool lowv = false) {
   double acc = 0.0;
-  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
       double rv = a.b[i] - ax;
       if (lowv) {
@@ -200,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
 
   // r = b - A x0 (FP64, and the FP32 copy the mixed recurrence starts from)
   double red[2] = {0.0, 0.0};
-  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
       const double bv = a.b[i];
       double rv = bv - ax;
@@ -241,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       if (infer && !force64) wave_infer(a.A, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
       infer = false;
       double gd[2] = {0.0, 0.0};
-      wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.r, a.r32, a.seg_p, lowv ? 2 : force64, wl[threadIdx.x >> 6],
+      solver_spmv<MODE, mixed>(a, a.r, a.r32, a.seg_p, lowv ? 2 : force64, wl[threadIdx.x >> 6],
                              lane, &cnt, [&](const int i, const double wv) {
         double rv;
         if (lowv) {
@@ -305,10 +365,10 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
         if (lowv) {
           for (int i = row0 + lane; i < row1; i += 64) a.p[i] = (double)a.p32[i];
           grid_sync(a.gs, epoch, &lds_i);
-          wave_spmv<kAll64, false>(a.A, tr0, tr1, a.p, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane,
+          solver_spmv<kAll64, false>(a, a.p, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane,
                                    nullptr, [&](const int i, const double sv) { a.s32[i] = (float)sv; });
         } else {
-          wave_spmv<kAll64, false>(a.A, tr0, tr1, a.p, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane,
+          solver_spmv<kAll64, false>(a, a.p, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane,
                                    nullptr, [&](const int i, const double sv) { a.s[i] = sv; });
         }
       }
@@ -465,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   double *v = a.q, *t = a.tv;
 
   double red[2] = {0.0, 0.0};
-  wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
       const double bv = a.b[i], rv = bv - ax;
       a.r[i] = rv;
@@ -512,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     double vr[1] = {0.0};
     if (infer && !force64) wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
     infer = false;
-    wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
+    solver_spmv<MODE, mixed>(a, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
       [&](const int i, const double vv) {
         v[i] = vv;
         vr[0] += vv * a.rh[i];
@@ -544,7 +604,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     grid_sync(a.gs, epoch, &lds_i);
     // t = A s, t.s and t.t
     double tw[2] = {0.0, 0.0};
-    wave_spmv<MODE, mixed>(a.A, tr0, tr1, a.s, a.s32, a.seg_s, force64, wl[threadIdx.x >> 6], lane, &cnt,
+    solver_spmv<MODE, mixed>(a, a.s, a.s32, a.seg_s, force64, wl[threadIdx.x >> 6], lane, &cnt,
       [&](const int i, const double tvv) {
         t[i] = tvv;
         tw[0] += tvv * a.s[i];
@@ -571,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
                             (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
       if (converged || periodic || omega == 0.0 || !isfinite(rr_new)) {
         double tr[2] = {0.0, 0.0};
-        wave_spmv<kAll64, false>(a.A, tr0, tr1, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+        solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
           [&](const int i, const double ax) {
             const double rv = a.b[i] - ax;
             a.r[i] = rv;
@@ -1009,22 +1069,125 @@ mf_status grid_for(K kernel, int override_blocks, int *G, int block = kBlock, bo
 
 // Per-wave tile-row ranges: nnz-balanced (memory-resident kernels) or an
 // even split of tile-rows (register-resident CG, whose per-wave state is
-// sized by rows).  Cached per (wave count, kind).
+// sized by rows), and each wave's SpMV descriptors.  A tile-row larger than
+// one window and than half a wave's share of the SpMV work is heavy: it is
+// cut into pieces spread over the waves with the least work, so hub rows of
+// power-law matrices do not serialise on one wave.  Cached per (wave count,
+// kind).
 mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
   const int key = 2 * nwaves + (even_rows ? 1 : 0);
   if (t->sched.wave_tr && t->sched_waves_for == key) return MF_OK;
-  if (t->sched.wave_tr) MF_HIP(hipFree(t->sched.wave_tr));
-  t->sched.wave_tr = nullptr;
+  for (void *p : {(void *)t->sched.wave_tr, (void *)t->sched.wave_win, (void *)t->sched.win_desc,
+                  (void *)t->sched.heavy_part, (void *)t->sched.heavy_cnt, (void *)t->sched.heavy_n,
+                  (void *)t->sched.heavy_base})
+    if (p) MF_HIP(hipFree(p));
+  t->sched = Schedule();
+  t->sched_waves_for = -1;
+  const int tm = t->tilem;
+  auto slots = [&](int r) { return t->pay_off[t->tile_ptr[r + 1]] - t->pay_off[t->tile_ptr[r]]; };
+  auto tiles = [&](int r) { return t->tile_ptr[r + 1] - t->tile_ptr[r]; };
+  auto cost = [&](int r) { return (double)slots(r) + 8.0 * tiles(r) + 16.0; };
+  std::vector<char> heavy(tm, 0);
   std::vector<int> b;
   if (even_rows) {
-    const int q = (t->tilem + nwaves - 1) / nwaves;
+    const int q = (tm + nwaves - 1) / nwaves;
     b.resize(nwaves + 1);
-    for (int w = 0; w <= nwaves; ++w) b[w] = std::min(t->tilem, w * q);
+    for (int w = 0; w <= nwaves; ++w) b[w] = std::min(tm, w * q);
   } else {
-    b = balance_tilerows(t, nwaves);
+    double total = 0.0;
+    for (int r = 0; r < tm; ++r) total += cost(r);
+    const double share = total / nwaves;
+    for (int r = 0; r < tm; ++r)
+      heavy[r] = (tiles(r) > kWinTiles || slots(r) > 4 * kWinChunks) && cost(r) > 0.5 * share;
+    // row ownership balanced on the work that stays with the owner
+    std::vector<double> pre(tm + 1, 0.0);
+    for (int r = 0; r < tm; ++r) pre[r + 1] = pre[r] + (heavy[r] ? 16.0 : cost(r));
+    b.assign(nwaves + 1, tm);
+    b[0] = 0;
+    int r = 0;
+    for (int w = 1; w < nwaves; ++w) {
+      const double target = pre[tm] * (double)w / (double)nwaves;
+      while (r < tm && pre[r] < target) ++r;
+      b[w] = std::max(r, b[w - 1]);
+    }
   }
+  // descriptors of each wave's non-heavy tile-rows, and the wave's load
+  std::vector<std::vector<WaveDesc>> per(nwaves);
+  std::vector<double> load(nwaves, 0.0);
+  for (int w = 0; w < nwaves; ++w) {
+    int r = b[w];
+    while (r < b[w + 1]) {
+      if (heavy[r]) { ++r; continue; }
+      int e = r;
+      while (e < b[w + 1] && !heavy[e]) load[w] += cost(e++);
+      window_partition_range(t, r, e, 4 * kWinChunks, per[w]);
+      r = e;
+    }
+  }
+  // heavy tile-rows: one-window pieces, each to the least loaded wave
+  // (a piece spans up to 8 windows, so a hub tile-row of 10^5 tiles is
+  // about 100 pieces)
+  constexpr int kPieceTiles = 8 * kWinTiles, kPieceSlots = 8 * 4 * kWinChunks;
+  std::vector<int> hn, hbase;
+  int npieces = 0;
+  std::priority_queue<std::pair<double, int>, std::vector<std::pair<double, int>>,
+                      std::greater<std::pair<double, int>>> pq;
+  for (int w = 0; w < nwaves; ++w) pq.push({load[w], w});
+  for (int r = 0; r < tm; ++r) {
+    if (!heavy[r]) continue;
+    const int h = (int)hn.size();
+    int np = 0;
+    int ta = t->tile_ptr[r];
+    const int tend = t->tile_ptr[r + 1];
+    while (ta < tend) {
+      int tb = ta;
+      while (tb < tend && tb - ta < kPieceTiles &&
+             t->pay_off[tb + 1] - t->pay_off[ta] <= kPieceSlots)
+        ++tb;
+      WaveDesc d;
+      d.tr0 = r;
+      d.tr1 = r + 1;
+      d.ta = ta;
+      d.T1 = tb;
+      d.Z0 = t->pay_off[ta];
+      d.Z1 = t->pay_off[tb];
+      d.flags = kDescPiece | (np << 8);
+      d.aux = h;
+      auto top = pq.top();
+      pq.pop();
+      per[top.second].push_back(d);
+      top.first += (double)(d.Z1 - d.Z0) + 8.0 * (tb - ta);
+      pq.push(top);
+      ++np;
+      ta = tb;
+    }
+    hn.push_back(np);
+    hbase.push_back(npieces);
+    npieces += np;
+  }
+  std::vector<int> ww(nwaves + 1, 0);
+  std::vector<WaveDesc> wd;
+  for (int w = 0; w < nwaves; ++w) {
+    wd.insert(wd.end(), per[w].begin(), per[w].end());
+    ww[w + 1] = (int)wd.size();
+  }
+  if (wd.empty()) wd.push_back(WaveDesc{0, 0, 0, 0, 0, 0, 0, 0});
+  const int nh = (int)hn.size();
   MF_HIP(hipMalloc(&t->sched.wave_tr, sizeof(int) * (nwaves + 1)));
   MF_HIP(hipMemcpy(t->sched.wave_tr, b.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice));
+  MF_HIP(hipMalloc(&t->sched.wave_win, sizeof(int) * (nwaves + 1)));
+  MF_HIP(hipMemcpy(t->sched.wave_win, ww.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice));
+  MF_HIP(hipMalloc(&t->sched.win_desc, sizeof(WaveDesc) * wd.size()));
+  MF_HIP(hipMemcpy(t->sched.win_desc, wd.data(), sizeof(WaveDesc) * wd.size(), hipMemcpyHostToDevice));
+  MF_HIP(hipMalloc(&t->sched.heavy_part, sizeof(double) * 16 * std::max(npieces, 1)));
+  MF_HIP(hipMalloc(&t->sched.heavy_cnt, sizeof(unsigned) * std::max(nh, 1)));
+  MF_HIP(hipMalloc(&t->sched.heavy_n, sizeof(int) * std::max(nh, 1)));
+  MF_HIP(hipMalloc(&t->sched.heavy_base, sizeof(int) * std::max(nh, 1)));
+  if (nh > 0) {
+    MF_HIP(hipMemcpy(t->sched.heavy_n, hn.data(), sizeof(int) * nh, hipMemcpyHostToDevice));
+    MF_HIP(hipMemcpy(t->sched.heavy_base, hbase.data(), sizeof(int) * nh, hipMemcpyHostToDevice));
+  }
+  t->sched.nheavy = nh;
   t->sched.nwaves = nwaves;
   t->sched_waves_for = key;
   return MF_OK;
@@ -1121,6 +1284,8 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
 
   const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
   MF_HIP(hipMemsetAsync(w.gs, 0, sizeof(GridSync), st));
+  if (t->sched.nheavy > 0)
+    MF_HIP(hipMemsetAsync(t->sched.heavy_cnt, 0, sizeof(unsigned) * t->sched.nheavy, st));
   MF_HIP(hipMemsetAsync(w.hist, 0, 5 * 8, st));
   MF_HIP(hipMemsetAsync(w.stats, 0, kStCount * 8, st));
   MF_HIP(hipMemsetAsync(w.seg_p, MF_PREC_FP64, segs, st));
@@ -1145,6 +1310,12 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   a.p32 = w.p32; a.s32 = w.s32; a.r32 = w.r32; a.q32 = w.q32;
   a.seg_p = w.seg_p; a.seg_s = w.seg_s;
   a.wave_tr = t->sched.wave_tr;
+  a.wave_win = t->sched.wave_win;
+  a.win_desc = t->sched.win_desc;
+  a.heavy_part = t->sched.heavy_part;
+  a.heavy_cnt = t->sched.heavy_cnt;
+  a.heavy_n = t->sched.heavy_n;
+  a.heavy_base = t->sched.heavy_base;
   a.partials = w.partials;
   a.gs = w.gs;
   a.stats = w.stats;

@@ -227,7 +227,8 @@ void free_device(mf_tiled *t) {
   void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.rp16, d.spmv_desc, d.tile_nz,
                   d.v64,      d.v32,      d.v16,      d.v8,       d.s16,    d.s8,
                   d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp, d.feat16,
-                  t->sched.wave_tr, t->work};
+                  t->sched.wave_tr, t->sched.wave_win, t->sched.win_desc, t->sched.heavy_part,
+                  t->sched.heavy_cnt, t->sched.heavy_n, t->sched.heavy_base, t->work};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   d = DeviceLayout();
