@@ -775,6 +775,64 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, int tr0, int tr1,
                                        cnt, epi);
 }
 
+// Heavy tile-rows cut into pieces: 16 partial row sums per piece, and per
+// heavy tile-row an arrival counter (zero between SpMVs), the piece count and
+// the first piece's slot.
+struct HeavyView {
+  double *part;
+  unsigned *cnt;
+  const int *n;
+  const int *base;
+};
+
+// One descriptor: a tile-row range through epi(row, value), or a piece of a
+// heavy tile-row.  Piece k of heavy tile-row h stores its 16 partial row sums
+// in its own slot (agent-scope atomic stores, visible across XCDs) and counts
+// itself done; the piece that completes the count (the last to finish, so no
+// wave ever waits for another) adds the slots in a fixed order
+// (deterministic), resets the counter for the next SpMV and runs epi for the
+// tile-row's 16 rows.
+template <int MODE, bool COUNT, int DEPTH, bool ONEWIN, bool FLAT, typename Epi>
+__device__ __forceinline__ void wave_spmv_piece(const TileView &A, const WaveDesc &desc,
+                                                const HeavyView &hv,
+                                                const double *__restrict__ x,
+                                                const float *__restrict__ x32,
+                                                const uint8_t *__restrict__ seglev, int force64,
+                                                WaveLds &L, int lane, PrecCount *cnt, Epi epi) {
+  const int flags = __builtin_amdgcn_readfirstlane(desc.flags);
+  const bool piece = flags & kDescPiece;
+  const int h = piece ? __builtin_amdgcn_readfirstlane(desc.aux) : 0;
+  const int base = piece ? __builtin_amdgcn_readfirstlane(hv.base[h]) : 0;
+  double *slot = hv.part + 16 * (size_t)(base + (flags >> 8));
+  wave_spmv<MODE, COUNT, DEPTH, ONEWIN, FLAT>(A, desc, x, x32, seglev, force64, L, lane, cnt,
+    [&](const int i, const double v) {
+      if (piece)
+        __hip_atomic_store(slot + (i & 15), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else
+        epi(i, v);
+    });
+  if (!piece) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  unsigned done = 0;
+  if (lane == 0)
+    done = __hip_atomic_fetch_add(hv.cnt + h, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+  done = __builtin_amdgcn_readfirstlane(done);
+  const int np = __builtin_amdgcn_readfirstlane(hv.n[h]);
+  if (done != (unsigned)np) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // lane (q, row): pieces q, q+4, ... of the row, then the four partial sums
+  // in q order
+  const int row = lane & 15, q = lane >> 4;
+  double sum = 0.0;
+  for (int k = q; k < np; k += 4)
+    sum += __hip_atomic_load(hv.part + 16 * (size_t)(base + k) + row, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  const double s1 = __shfl(sum, row + 16), s2 = __shfl(sum, row + 32), s3 = __shfl(sum, row + 48);
+  const int R = __builtin_amdgcn_readfirstlane(desc.tr0);
+  if (lane < 16) epi(16 * R + lane, ((sum + s1) + s2) + s3);
+  if (lane == 0) __hip_atomic_store(hv.cnt + h, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------------------
 // Row-group tiled SpMV.
 //

@@ -208,9 +208,49 @@ void window_partition_range(const mf_tiled *t, int r0, int r1, int target_nnz,
   }
 }
 
-std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz) {
+int cut_pieces(const mf_tiled *t, int r, int h, int max_tiles, int max_slots, int extra_flags,
+               std::vector<WaveDesc> &out) {
+  int np = 0;
+  int ta = t->tile_ptr[r];
+  const int tend = t->tile_ptr[r + 1];
+  while (ta < tend) {
+    int tb = ta + 1;  // a tile holds at most 256 slots
+    while (tb < tend && tb - ta < max_tiles && t->pay_off[tb + 1] - t->pay_off[ta] <= max_slots) ++tb;
+    WaveDesc d;
+    d.tr0 = r;
+    d.tr1 = r + 1;
+    d.ta = ta;
+    d.T1 = tb;
+    d.Z0 = t->pay_off[ta];
+    d.Z1 = t->pay_off[tb];
+    d.flags = kDescPiece | extra_flags | (np << 8);
+    d.aux = h;
+    out.push_back(d);
+    ++np;
+    ta = tb;
+  }
+  return np;
+}
+
+std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz, std::vector<int> *hn,
+                                       std::vector<int> *hbase) {
   std::vector<WaveDesc> out;
-  window_partition_range(t, 0, t->tilem, target_nnz, out);
+  const int heavy_slots = 4 * std::max(target_nnz, 4 * kWinChunks);
+  int r = 0, nh = 0, npieces = 0;
+  while (r < t->tilem) {
+    int e = r;
+    while (e < t->tilem &&
+           t->pay_off[t->tile_ptr[e + 1]] - t->pay_off[t->tile_ptr[e]] <= heavy_slots)
+      ++e;
+    window_partition_range(t, r, e, target_nnz, out);
+    if (e == t->tilem) break;
+    const int np = cut_pieces(t, e, nh, kWinTiles, 4 * kWinChunks, kDescOneWindow, out);
+    if (hn) hn->push_back(np);
+    if (hbase) hbase->push_back(npieces);
+    npieces += np;
+    ++nh;
+    r = e + 1;
+  }
   return out;
 }
 

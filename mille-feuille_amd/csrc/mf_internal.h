@@ -70,10 +70,18 @@ struct DeviceLayout {
   void *mlp = nullptr;
   uint16_t *feat16 = nullptr;
   int has_mlp = 0;
-  // stand-alone SpMV schedule: one descriptor per wave (window_partition)
+  // stand-alone SpMV schedule: one descriptor per wave (window_partition),
+  // heavy tile-rows cut into one-window pieces whose row sums the last piece
+  // to finish adds (16 partial sums per piece; per heavy tile-row a counter,
+  // the piece count and the first piece's slot)
   WaveDesc *spmv_desc = nullptr;
   int spmv_nwaves = 0;
   int spmv_wave_nnz = 0;
+  double *spmv_part = nullptr;
+  unsigned *spmv_cnt = nullptr;
+  int *spmv_hn = nullptr;
+  int *spmv_hbase = nullptr;
+  int spmv_nheavy = 0;
 };
 
 struct Schedule {
@@ -131,7 +139,18 @@ std::vector<int> balance_tilerows(const mf_tiled *t, int nwaves);
 // one SpMV window (<= kWinRows tile-rows, <= kWinTiles tiles, <= kWinChunks
 // chunks) unless a single tile-row is larger than a window (such a tile-row
 // gets a wave of its own).  Needs pay_off.
-std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz);
+// Stand-alone SpMV schedule: descriptors of about target_nnz payload slots,
+// except that a tile-row of more than 4 * max(target_nnz, 4 * kWinChunks)
+// slots is heavy and cut into one-window pieces (kDescPiece); hn and hbase
+// receive each heavy tile-row's piece count and first piece slot.
+std::vector<WaveDesc> window_partition(const mf_tiled *t, int target_nnz,
+                                       std::vector<int> *hn = nullptr,
+                                       std::vector<int> *hbase = nullptr);
+// Pieces of tile-row r (heavy tile-row index h) of at most max_tiles tiles and
+// max_slots payload slots, appended to out with kDescPiece | extra_flags;
+// returns their number.
+int cut_pieces(const mf_tiled *t, int r, int h, int max_tiles, int max_slots, int extra_flags,
+               std::vector<WaveDesc> &out);
 // Tile-rows [r0, r1) cut into descriptors of at most target_nnz payload
 // slots, each within one window's limits unless a single tile-row exceeds
 // them (such a descriptor walks several windows).

@@ -136,12 +136,7 @@ __device__ __forceinline__ void flush_hist(const PrecCount &c, unsigned long lon
 }
 
 // SpMV of the calling wave's share: its descriptors (whole tile-rows) with
-// epi(row, value), then its pieces of heavy tile-rows.  Piece k of heavy
-// tile-row h stores its 16 partial row sums in its own slot (agent-scope
-// atomic stores) and counts itself done; the piece that completes the count
-// (the last to finish, so no wave ever waits for another) adds the slots in
-// a fixed order (deterministic), resets the counter for the next SpMV and
-// runs epi for the 16 rows.
+// epi(row, value), then its pieces of heavy tile-rows (wave_spmv_piece).
 template <int MODE, bool COUNT, typename Epi>
 __device__ __forceinline__ void solver_spmv(const SolveArgs &a, const double *__restrict__ x,
                                             const float *__restrict__ x32,
@@ -150,42 +145,10 @@ __device__ __forceinline__ void solver_spmv(const SolveArgs &a, const double *__
   const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const int d0 = __builtin_amdgcn_readfirstlane(a.wave_win[wid]);
   const int d1 = __builtin_amdgcn_readfirstlane(a.wave_win[wid + 1]);
-  for (int d = d0; d < d1; ++d) {
-    const WaveDesc &desc = a.win_desc[d];
-    const int flags = __builtin_amdgcn_readfirstlane(desc.flags);
-    const bool piece = flags & kDescPiece;
-    const int h = piece ? __builtin_amdgcn_readfirstlane(desc.aux) : 0;
-    const int base = piece ? __builtin_amdgcn_readfirstlane(a.heavy_base[h]) : 0;
-    double *slot = a.heavy_part + 16 * (size_t)(base + (flags >> 8));
-    wave_spmv<MODE, COUNT, 3, false>(a.A, desc, x, x32, seglev, force64, L, lane, cnt,
-      [&](const int i, const double v) {
-        if (piece)
-          __hip_atomic_store(slot + (i & 15), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else
-          epi(i, v);
-      });
-    if (!piece) continue;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    unsigned done = 0;
-    if (lane == 0)
-      done = __hip_atomic_fetch_add(a.heavy_cnt + h, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
-    done = __builtin_amdgcn_readfirstlane(done);
-    const int np = __builtin_amdgcn_readfirstlane(a.heavy_n[h]);
-    if (done == (unsigned)np) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      // lane (q, row): pieces q, q+4, ... of the row, then the four partial
-      // sums in q order
-      const int row = lane & 15, q = lane >> 4;
-      double sum = 0.0;
-      for (int k = q; k < np; k += 4)
-        sum += __hip_atomic_load(a.heavy_part + 16 * (size_t)(base + k) + row, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-      const double s1 = __shfl(sum, row + 16), s2 = __shfl(sum, row + 32), s3 = __shfl(sum, row + 48);
-      const int R = __builtin_amdgcn_readfirstlane(desc.tr0);
-      if (lane < 16) epi(16 * R + lane, ((sum + s1) + s2) + s3);
-      if (lane == 0) __hip_atomic_store(a.heavy_cnt + h, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  const HeavyView hv{a.heavy_part, a.heavy_cnt, a.heavy_n, a.heavy_base};
+  for (int d = d0; d < d1; ++d)
+    wave_spmv_piece<MODE, COUNT, 3, false, false>(a.A, a.win_desc[d], hv, x, x32, seglev, force64,
+                                                  L, lane, cnt, epi);
 }
 
 // True residual of the wave's tile-rows: r_new = b - A x (FP64), optionally
@@ -1136,30 +1099,14 @@ mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
   for (int r = 0; r < tm; ++r) {
     if (!heavy[r]) continue;
     const int h = (int)hn.size();
-    int np = 0;
-    int ta = t->tile_ptr[r];
-    const int tend = t->tile_ptr[r + 1];
-    while (ta < tend) {
-      int tb = ta;
-      while (tb < tend && tb - ta < kPieceTiles &&
-             t->pay_off[tb + 1] - t->pay_off[ta] <= kPieceSlots)
-        ++tb;
-      WaveDesc d;
-      d.tr0 = r;
-      d.tr1 = r + 1;
-      d.ta = ta;
-      d.T1 = tb;
-      d.Z0 = t->pay_off[ta];
-      d.Z1 = t->pay_off[tb];
-      d.flags = kDescPiece | (np << 8);
-      d.aux = h;
+    std::vector<WaveDesc> pcs;
+    const int np = cut_pieces(t, r, h, kPieceTiles, kPieceSlots, 0, pcs);
+    for (const WaveDesc &d : pcs) {
       auto top = pq.top();
       pq.pop();
       per[top.second].push_back(d);
-      top.first += (double)(d.Z1 - d.Z0) + 8.0 * (tb - ta);
+      top.first += (double)(d.Z1 - d.Z0) + 8.0 * (d.T1 - d.ta);
       pq.push(top);
-      ++np;
-      ta = tb;
     }
     hn.push_back(np);
     hbase.push_back(npieces);

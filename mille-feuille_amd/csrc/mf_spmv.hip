@@ -18,9 +18,11 @@ namespace mf {
 // 64 VGPRs (8 waves per SIMD; 32 waves x 4.75 KiB of LDS per CU) for FP64
 // payloads and 80 (6 waves) for per-tile levels, the most that compile
 // without spilling.
-template <int MODE>
+// PIECES: the schedule has heavy tile-rows cut into pieces (wave_spmv_piece);
+// without them the kernel keeps the piece bookkeeping out of its registers.
+template <int MODE, bool PIECES>
 __global__ __launch_bounds__(256, MODE == kAll64 ? 8 : 6) void k_spmv(TileView A, const WaveDesc *__restrict__ desc,
-                                              int nwaves, const double *__restrict__ x,
+                                              int nwaves, HeavyView hv, const double *__restrict__ x,
                                               const float *__restrict__ x32,
                                               double *__restrict__ y) {
   __shared__ WaveLds wl[4];
@@ -29,8 +31,12 @@ __global__ __launch_bounds__(256, MODE == kAll64 ? 8 : 6) void k_spmv(TileView A
   const int w = __builtin_amdgcn_readfirstlane(lb * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (w >= nwaves) return;
-  wave_spmv<MODE, false>(A, desc[w], x, x32, nullptr, 0, wl[threadIdx.x >> 6],
-                         lane, nullptr, [&](const int i, const double v) { y[i] = v; });
+  auto store = [&](const int i, const double v) { y[i] = v; };
+  if (PIECES)
+    wave_spmv_piece<MODE, false, 3, true, true>(A, desc[w], hv, x, x32, nullptr, 0,
+                                                wl[threadIdx.x >> 6], lane, nullptr, store);
+  else
+    wave_spmv<MODE, false>(A, desc[w], x, x32, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr, store);
 }
 
 // Row-group kernel: four consecutive tile-rows per wave, one row per lane,
@@ -71,12 +77,19 @@ mf_status launch_spmv(const mf_tiled *t, const double *d_x, const float *d_x32, 
     }
   } else {
     const int nw = t->dev.spmv_nwaves;
+    const HeavyView hv{t->dev.spmv_part, t->dev.spmv_cnt, t->dev.spmv_hn, t->dev.spmv_hbase};
     const int blocks = ((nw + 3) / 4 + 7) / 8 * 8;
     if (A.tilem > 0 && nw > 0) {
-      if (mode == kAll64)
-        k_spmv<kAll64><<<blocks, 256, 0, s>>>(A, t->dev.spmv_desc, nw, d_x, d_x32, d_y);
+      const bool pieces = t->dev.spmv_nheavy > 0;
+      const WaveDesc *desc = t->dev.spmv_desc;
+      if (mode == kAll64 && pieces)
+        k_spmv<kAll64, true><<<blocks, 256, 0, s>>>(A, desc, nw, hv, d_x, d_x32, d_y);
+      else if (mode == kAll64)
+        k_spmv<kAll64, false><<<blocks, 256, 0, s>>>(A, desc, nw, hv, d_x, d_x32, d_y);
+      else if (pieces)
+        k_spmv<kPerTile, true><<<blocks, 256, 0, s>>>(A, desc, nw, hv, d_x, d_x32, d_y);
       else
-        k_spmv<kPerTile><<<blocks, 256, 0, s>>>(A, t->dev.spmv_desc, nw, d_x, d_x32, d_y);
+        k_spmv<kPerTile, false><<<blocks, 256, 0, s>>>(A, desc, nw, hv, d_x, d_x32, d_y);
       MF_HIP(hipGetLastError());
     }
   }

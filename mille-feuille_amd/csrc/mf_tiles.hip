@@ -100,6 +100,8 @@ static mf_status dalloc(T **p, size_t n) {
 }
 
 
+mf_status upload_spmv_schedule(mf_tiled *t, int wave_nnz);
+
 mf_status upload_tiles(mf_tiled *t, int device) {
   DeviceLayout &d = t->dev;
   MF_HIP(hipSetDevice(device));
@@ -156,11 +158,6 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   }
   d.max_tiles_per_row = max_tiles;
   d.spmv_kernel = default_spmv_kernel(t);
-  // per-wave work of the stand-alone SpMV, about kSpmvWaveNnz entries per
-  // wave (retuned by mf_tile_set_schedule)
-  const std::vector<WaveDesc> wdesc = window_partition(t, kSpmvWaveNnz);
-  const int nw = (int)wdesc.size();
-  d.spmv_wave_nnz = kSpmvWaveNnz;
   MF_TRY(dalloc(&d.tile_ptr, TM + 1));
   MF_TRY(dalloc(&d.tile_col, T));
   MF_TRY(dalloc(&d.tile_off, T + 1));
@@ -172,8 +169,6 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_TRY(dalloc(&d.v32, (size_t)ZP + kPad));
   MF_TRY(dalloc(&d.v16, (size_t)ZP + kPad));
   MF_TRY(dalloc(&d.v8, (size_t)ZP + kPad));
-  if (nw > 0) MF_TRY(dalloc(&d.spmv_desc, nw));
-  d.spmv_nwaves = nw;
   MF_TRY(dalloc(&d.s16, T));
   MF_TRY(dalloc(&d.s8, T));
   MF_TRY(dalloc(&d.qerr, 3 * (size_t)T));
@@ -191,9 +186,10 @@ mf_status upload_tiles(mf_tiled *t, int device) {
   MF_HIP(hipMemset(d.v16, 0, sizeof(uint16_t) * ((size_t)ZP + kPad)));
   MF_HIP(hipMemset(d.v8, 0, (size_t)ZP + kPad));
   MF_HIP(hipMemcpy(d.v64, v64.data(), sizeof(double) * ((size_t)ZP + kPad), hipMemcpyHostToDevice));
-  if (nw > 0)
-    MF_HIP(hipMemcpy(d.spmv_desc, wdesc.data(), sizeof(WaveDesc) * nw, hipMemcpyHostToDevice));
   MF_HIP(hipMemcpy(d.diag_tile, diag.data(), sizeof(int) * TM, hipMemcpyHostToDevice));
+  // per-wave work of the stand-alone SpMV, about kSpmvWaveNnz entries per
+  // wave (retuned by mf_tile_set_schedule)
+  MF_TRY(upload_spmv_schedule(t, kSpmvWaveNnz));
   MF_HIP(hipMemset(d.prec, MF_PREC_FP64, T));
   MF_HIP(hipMemset(d.prec_user, MF_PREC_FP64, T));
   if (T > 0) {
@@ -221,10 +217,47 @@ int default_spmv_kernel(const mf_tiled *) {
   return MF_SPMV_STREAM;
 }
 
+// Stand-alone SpMV schedule of about wave_nnz payload slots per wave, heavy
+// tile-rows cut into one-window pieces, uploaded in place of the previous one.
+mf_status upload_spmv_schedule(mf_tiled *t, int wave_nnz) {
+  DeviceLayout &d = t->dev;
+  std::vector<int> hn, hbase;
+  const std::vector<WaveDesc> wd = window_partition(t, wave_nnz, &hn, &hbase);
+  const int nw = (int)wd.size(), nh = (int)hn.size();
+  const int npieces = nh > 0 ? hbase.back() + hn.back() : 0;
+  for (void *p : {(void *)d.spmv_desc, (void *)d.spmv_part, (void *)d.spmv_cnt, (void *)d.spmv_hn,
+                  (void *)d.spmv_hbase})
+    if (p) MF_HIP(hipFree(p));
+  d.spmv_desc = nullptr;
+  d.spmv_part = nullptr;
+  d.spmv_cnt = nullptr;
+  d.spmv_hn = d.spmv_hbase = nullptr;
+  d.spmv_nwaves = 0;
+  d.spmv_nheavy = 0;
+  if (nw > 0) {
+    MF_TRY(dalloc(&d.spmv_desc, nw));
+    MF_HIP(hipMemcpy(d.spmv_desc, wd.data(), sizeof(WaveDesc) * nw, hipMemcpyHostToDevice));
+  }
+  MF_TRY(dalloc(&d.spmv_part, 16 * (size_t)std::max(npieces, 1)));
+  MF_TRY(dalloc(&d.spmv_cnt, std::max(nh, 1)));
+  MF_TRY(dalloc(&d.spmv_hn, std::max(nh, 1)));
+  MF_TRY(dalloc(&d.spmv_hbase, std::max(nh, 1)));
+  MF_HIP(hipMemset(d.spmv_cnt, 0, sizeof(unsigned) * std::max(nh, 1)));
+  if (nh > 0) {
+    MF_HIP(hipMemcpy(d.spmv_hn, hn.data(), sizeof(int) * nh, hipMemcpyHostToDevice));
+    MF_HIP(hipMemcpy(d.spmv_hbase, hbase.data(), sizeof(int) * nh, hipMemcpyHostToDevice));
+  }
+  d.spmv_nwaves = nw;
+  d.spmv_nheavy = nh;
+  d.spmv_wave_nnz = wave_nnz;
+  return MF_OK;
+}
+
 void free_device(mf_tiled *t) {
   DeviceLayout &d = t->dev;
   if (d.device >= 0) (void)hipSetDevice(d.device);
   void *ptrs[] = {d.tile_ptr, d.tile_col, d.tile_off, d.tile_row, d.rc, d.rp16, d.spmv_desc, d.tile_nz,
+                  d.spmv_part, d.spmv_cnt, d.spmv_hn, d.spmv_hbase,
                   d.v64,      d.v32,      d.v16,      d.v8,       d.s16,    d.s8,
                   d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp, d.feat16,
                   t->sched.wave_tr, t->sched.wave_win, t->sched.win_desc, t->sched.heavy_part,
@@ -323,21 +356,7 @@ extern "C" mf_status mf_tile_set_schedule(mf_tiled *t, int spmv_wave_nnz) {
   if (spmv_wave_nnz < 64 || spmv_wave_nnz > (1 << 20))
     return fail(MF_ERR_ARG, "mf_tile_set_schedule: spmv_wave_nnz out of [64, 2^20]");
   MF_HIP(hipSetDevice(t->dev.device));
-  const std::vector<WaveDesc> wdesc = window_partition(t, spmv_wave_nnz);
-  const int nw = (int)wdesc.size();
-  WaveDesc *p = nullptr;
-  if (nw > 0) {
-    MF_HIP(hipMalloc(&p, sizeof(WaveDesc) * nw));
-    hipError_t e = hipMemcpy(p, wdesc.data(), sizeof(WaveDesc) * nw, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      (void)hipFree(p);
-      return fail(MF_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-    }
-  }
   MF_HIP(hipDeviceSynchronize());
-  if (t->dev.spmv_desc) MF_HIP(hipFree(t->dev.spmv_desc));
-  t->dev.spmv_desc = p;
-  t->dev.spmv_nwaves = nw;
-  t->dev.spmv_wave_nnz = spmv_wave_nnz;
+  MF_TRY(upload_spmv_schedule(t, spmv_wave_nnz));
   return MF_OK;
 }

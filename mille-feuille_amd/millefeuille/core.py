@@ -161,6 +161,8 @@ class TiledMatrix:
 
     def set_schedule(self, spmv_wave_nnz):
         check("mf_tile_set_schedule", lib.mf_tile_set_schedule(self.h, int(spmv_wave_nnz)))
+        if int(spmv_wave_nnz) > 0:
+            self._wave_nnz = int(spmv_wave_nnz)
 
     def tune_spmv(self, mode=0):
         """Autotunes the stand-alone SpMV (mf_tune_spmv) for `mode` at the
@@ -170,11 +172,23 @@ class TiledMatrix:
         check("mf_tune_spmv", lib.mf_tune_spmv(self.h, int(mode), C.byref(rep)))
         d = rep.as_dict()
         self._spmv_kernel = _lib.SPMV_ROWGROUP if d["best"] == 0 else _lib.SPMV_STREAM
+        if d["best"] > 0:
+            self._wave_nnz = int(d["best"])
         return d
 
     def spmv_kernel_name(self):
-        """Device kernel that mf_spmv launches for this handle (mode 0)."""
-        return "mf::k_spmv_rg<0>" if self._spmv_kernel == _lib.SPMV_ROWGROUP else "mf::k_spmv<0>"
+        """Device kernel that mf_spmv launches for this handle (mode 0): the
+        row-group kernel, or the streamed one, instantiated with piece
+        bookkeeping when a tile-row holds more than 4*max(wave_nnz, 4096)
+        padded payload slots (the library's heavy tile-row rule)."""
+        if self._spmv_kernel == _lib.SPMV_ROWGROUP:
+            return "mf::k_spmv_rg<0>"
+        a = self.arrays()
+        slots = (np.diff(a["csr_offset"]).astype(np.int64) + 3) // 4 * 4
+        pre = np.concatenate([[0], np.cumsum(slots)])
+        row_slots = np.diff(pre[np.asarray(a["tile_ptr"], np.int64)])
+        heavy = bool((row_slots > 4 * max(getattr(self, "_wave_nnz", 1024), 4096)).any())
+        return f"mf::k_spmv<0, {'true' if heavy else 'false'}>"
 
     def set_spmv_kernel(self, kind):
         """'auto', 'stream' or 'rowgroup' (or the MF_SPMV_* value)."""
