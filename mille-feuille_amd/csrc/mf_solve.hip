@@ -33,6 +33,7 @@ const MlpDevice *policy_mlp(const mf_tiled *t);
 
 constexpr int kBlock = 1024;  // 16 waves, one block per CU
 constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kUpd = 4;  // 64-row groups per step of the vector loops (loads first)
 
 enum StatSlot {
   kStIters = 0,
@@ -151,11 +152,12 @@ __device__ __forceinline__ void solver_spmv(const SolveArgs &a, const double *__
                                                   L, lane, cnt, epi);
 }
 
-// True residual of the wave's tile-rows: r_new = b - A x (FP64), optionally
+// True residual of the wave's SpMV share (its tile-rows, and the heavy
+// tile-rows whose last piece it completes): r_new = b - A x (FP64), optionally
 // stored in FP64 (store) or FP32 (store32).  Returns the wave's share of
 // ||r_new||^2 (of the stored FP32 values when store32 is given).
-__device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int tr0, int tr1,
-                                                     int lane, double *store, WaveLds *wl,
+__device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int lane, double *store,
+                                                     WaveLds *wl,
                                                      float *store32 = nullptr) {
   double acc = 0.0;
   solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
@@ -172,12 +174,11 @@ __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int tr0
   return acc;
 }
 
-// Restart helper: r = b - A x in FP64 for the wave's tile-rows, p = r (and
+// Restart helper: r = b - A x in FP64 for the wave's SpMV share, p = r (and
 // r^ = r for BiCGSTAB).  Returns the wave's share of ||r||^2.
 // lowv: the mixed CG's FP32 vectors r32 = p32 = (float) r instead.
-__device__ __forceinline__ double restart_rows(const SolveArgs &a, int tr0, int tr1, int lane,
-                                               bool mixed, bool bicg, WaveLds *wl,
-                                               bool lowv = false) {
+__device__ __forceinline__ double restart_rows(const SolveArgs &a, int lane, bool mixed, bool bicg,
+                                               WaveLds *wl, bool lowv = false) {
   double acc = 0.0;
   solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
     [&](const int i, const double ax) {
@@ -294,8 +295,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       // FP64 residual replacement r = b - A x (x is complete: the last
       // update ended with a grid sync)
       const bool lv = lowv && !force64;
-      double tr[1] = {lv ? true_residual_rows(a, tr0, tr1, lane, nullptr, wl, a.r32)
-                         : true_residual_rows(a, tr0, tr1, lane, a.r, wl)};
+      double tr[1] = {lv ? true_residual_rows(a, lane, nullptr, wl, a.r32)
+                         : true_residual_rows(a, lane, a.r, wl)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
       it_ref = it;
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
         // the FP64 recurrence starts from an FP64 residual
         lowv = false;
         if (lv) {
-          double t2[1] = {true_residual_rows(a, tr0, tr1, lane, a.r, wl)};
+          double t2[1] = {true_residual_rows(a, lane, a.r, wl)};
           grid_sum<1>(t2, a.partials, a.gs, epoch, slot, lds, &lds_i);
           rr = t2[0];
         }
@@ -371,14 +372,13 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
     const bool lupd = learned && !force64 && (it % a.check_interval) == 0;
     // Four 64-row groups per step, every load of a step issued before its
     // arithmetic, so a wave keeps 20 loads in flight instead of 5.
-    constexpr int kU = 4;
-    for (int base = row0; base < row1; base += 64 * kU) {
-      double rv[kU], rn[kU];
+    for (int base = row0; base < row1; base += 64 * kUpd) {
+      double rv[kUpd], rn[kUpd];
       if (lowv) {
-        float r_[kU], p_[kU], q_[kU], s_[kU];
-        double x_[kU];
+        float r_[kUpd], p_[kUpd], q_[kUpd], s_[kUpd];
+        double x_[kUpd];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUpd; ++u) {
           const int i = base + 64 * u + lane;
           const bool ok = i < row1;
           r_[u] = ok ? a.r32[i] : 0.0f;
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           x_[u] = ok ? a.x[i] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUpd; ++u) {
           const int i = base + 64 * u + lane;
           rv[u] = r_[u];
           const double pv = rv[u] + beta * (double)p_[u];
@@ -404,9 +404,9 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           }
         }
       } else {
-        double r_[kU], p_[kU], q_[kU], s_[kU], x_[kU];
+        double r_[kUpd], p_[kUpd], q_[kUpd], s_[kUpd], x_[kUpd];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUpd; ++u) {
           const int i = base + 64 * u + lane;
           const bool ok = i < row1;
           r_[u] = ok ? a.r[i] : 0.0;
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           x_[u] = ok ? a.x[i] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
+        for (int u = 0; u < kUpd; ++u) {
           const int i = base + 64 * u + lane;
           rv[u] = r_[u];
           const double pv = rv[u] + beta * p_[u];
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
         }
       }
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
+      for (int u = 0; u < kUpd; ++u) {
         const int i = base + 64 * u + lane;
         double d = i < row1 ? (lupd ? rn[u] * rn[u] : fabs(rn[u] - rv[u])) : 0.0;
         if (upd) {
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   }
   if (mixed) flush_hist(cnt, a.hist, lds);
   // Final FP64 residual for the report.
-  double fin[1] = {true_residual_rows(a, tr0, tr1, lane, nullptr, wl)};
+  double fin[1] = {true_residual_rows(a, lane, nullptr, wl)};
   grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.stats[kStIters] = it;
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   while (!done) {
     if (restart) {
       if (++restarts > kMaxRestart) { status = MF_ERR_BREAKDOWN; break; }
-      double tr[1] = {restart_rows(a, tr0, tr1, lane, mixed, true, wl)};
+      double tr[1] = {restart_rows(a, lane, mixed, true, wl)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
       rr = rho = tr[0];
@@ -549,19 +549,30 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     const double alpha = uniform_double(rho / vr[0]);
     // s = r - alpha v (rule policy: |s_new - s_old| per segment)
     const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
-    for (int base = row0; base < row1; base += 64) {
-      const int i = base + lane;
-      double d = 0.0;
-      if (i < row1) {
-        const double so = a.s[i];
-        const double sv = a.r[i] - alpha * v[i];
-        a.s[i] = sv;
-        if (mixed) a.s32[i] = (float)sv;
-        d = fabs(sv - so);
+    for (int base0 = row0; base0 < row1; base0 += 64 * kUpd) {
+      double so[kUpd], ro[kUpd], vo[kUpd];
+#pragma unroll
+      for (int u = 0; u < kUpd; ++u) {
+        const int i = base0 + 64 * u + lane;
+        const bool ok = i < row1;
+        so[u] = ok ? a.s[i] : 0.0;
+        ro[u] = ok ? a.r[i] : 0.0;
+        vo[u] = ok ? v[i] : 0.0;
       }
-      if (upd) {
-        d = group16_max(d);
-        if ((lane & 15) == 0 && i < row1) a.seg_s[i >> 4] = (uint8_t)rule_level(d);
+#pragma unroll
+      for (int u = 0; u < kUpd; ++u) {
+        const int i = base0 + 64 * u + lane;
+        double d = 0.0;
+        if (i < row1) {
+          const double sv = ro[u] - alpha * vo[u];
+          a.s[i] = sv;
+          if (mixed) a.s32[i] = (float)sv;
+          d = fabs(sv - so[u]);
+        }
+        if (upd) {
+          d = group16_max(d);
+          if ((lane & 15) == 0 && i < row1) a.seg_s[i >> 4] = (uint8_t)rule_level(d);
+        }
       }
     }
     grid_sync(a.gs, epoch, &lds_i);
@@ -577,12 +588,30 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     const double omega = uniform_double(tw[1] != 0.0 && isfinite(tw[1]) ? tw[0] / tw[1] : 0.0);
     // x += alpha p + omega s, r = s - omega t, rho' = r.rh, ||r||^2
     double nw[2] = {0.0, 0.0};
-    for (int i = row0 + lane; i < row1; i += 64) {
-      a.x[i] += alpha * a.p[i] + omega * a.s[i];
-      const double rv = a.s[i] - omega * t[i];
-      a.r[i] = rv;
-      nw[0] += rv * a.rh[i];
-      nw[1] += rv * rv;
+    // four 64-row groups per step, loads first (20 in flight per lane)
+    for (int base = row0 + lane; base < row1; base += 64 * kUpd) {
+      double x_[kUpd], p_[kUpd], s_[kUpd], t_[kUpd], h_[kUpd];
+#pragma unroll
+      for (int u = 0; u < kUpd; ++u) {
+        const int i = base + 64 * u;
+        const bool ok = i < row1;
+        x_[u] = ok ? a.x[i] : 0.0;
+        p_[u] = ok ? a.p[i] : 0.0;
+        s_[u] = ok ? a.s[i] : 0.0;
+        t_[u] = ok ? t[i] : 0.0;
+        h_[u] = ok ? a.rh[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUpd; ++u) {
+        const int i = base + 64 * u;
+        if (i < row1) {
+          a.x[i] = x_[u] + (alpha * p_[u] + omega * s_[u]);
+          const double rv = s_[u] - omega * t_[u];
+          a.r[i] = rv;
+          nw[0] += rv * h_[u];
+          nw[1] += rv * rv;
+        }
+      }
     }
     grid_sum<2>(nw, a.partials, a.gs, epoch, slot, lds, &lds_i);
     double rho_new = nw[0], rr_new = nw[1];
@@ -626,24 +655,35 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     const bool updp = MODE == kPerSeg && (it % a.check_interval) == 0;
     const bool lupd = learned && !force64 && (it % a.check_interval) == 0;
     double pp[1] = {0.0};
-    for (int base = row0; base < row1; base += 64) {
-      const int i = base + lane;
-      double d = 0.0;
-      if (i < row1) {
-        const double po = a.p[i];
-        const double pv = a.r[i] + beta * (po - omega * v[i]);
-        a.p[i] = pv;
-        if (mixed) a.p32[i] = (float)pv;
-        d = lupd ? pv * pv : fabs(pv - po);
+    for (int base0 = row0; base0 < row1; base0 += 64 * kUpd) {
+      double po[kUpd], ro[kUpd], vo[kUpd];
+#pragma unroll
+      for (int u = 0; u < kUpd; ++u) {
+        const int i = base0 + 64 * u + lane;
+        const bool ok = i < row1;
+        po[u] = ok ? a.p[i] : 0.0;
+        ro[u] = ok ? a.r[i] : 0.0;
+        vo[u] = ok ? v[i] : 0.0;
       }
-      if (updp) {
-        d = group16_max(d);
-        if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
-      }
-      if (lupd) {
-        pp[0] += d;
-        d = group16_sum(d);
-        if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d;
+#pragma unroll
+      for (int u = 0; u < kUpd; ++u) {
+        const int i = base0 + 64 * u + lane;
+        double d = 0.0;
+        if (i < row1) {
+          const double pv = ro[u] + beta * (po[u] - omega * vo[u]);
+          a.p[i] = pv;
+          if (mixed) a.p32[i] = (float)pv;
+          d = lupd ? pv * pv : fabs(pv - po[u]);
+        }
+        if (updp) {
+          d = group16_max(d);
+          if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
+        }
+        if (lupd) {
+          pp[0] += d;
+          d = group16_sum(d);
+          if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d;
+        }
       }
     }
     if (lupd) {
@@ -655,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     }
   }
   if (mixed) flush_hist(cnt, a.hist, lds);
-  double fin[1] = {true_residual_rows(a, tr0, tr1, lane, nullptr, wl)};
+  double fin[1] = {true_residual_rows(a, lane, nullptr, wl)};
   grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.stats[kStIters] = it;
