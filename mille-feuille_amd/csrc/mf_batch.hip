@@ -147,8 +147,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cg(BatchArgs a) {
       }
       ++it;
       double pq0 = 0.0;
-      wave_spmv<kPerTile, false>(A, tr0, tr1, p, p32, nullptr, force64, L, lane, &cnt,
-        [&](const int i, const double qv) {
+      wave_spmv<kPerTile, false, 3, false, true>(A, wave_desc(A, tr0, tr1), p, p32, nullptr, force64, L,
+                                                 lane, &cnt, [&](const int i, const double qv) {
           q[i] = qv;
           pq0 += p[i] * qv;
         });

@@ -596,9 +596,8 @@ __device__ __forceinline__ void window_slices(const TileView &A, WaveLds &L, int
 // slice's (row,col) bytes.  Other descriptors walk their tile-rows in windows
 // that the limits cut, a tile-row's partial sum carried across a cut.
 // ONEWIN = false compiles the walk only, for callers whose descriptors never
-// carry the flag.  FLAT selects the branch-free per-level consume; kernels
-// whose loop state shares the register budget (the solvers) use the
-// branching one, which keeps fewer values live.
+// carry the flag.  FLAT selects the branch-free per-level consume; the
+// branching one keeps fewer values live, for kernels short of registers.
 template <int MODE, bool COUNT, int DEPTH = 3, bool ONEWIN = true, bool FLAT = ONEWIN,
           typename Epi>
 __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &desc,
