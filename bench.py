@@ -8,7 +8,11 @@ from seed 2, b ~ U[-1,1] from seed 2, x0 = 0, tolerance 1e-10.
 
 A step is one complete solve (x reset to 0, one persistent-kernel launch, to
 the tolerance) with the matrix and b already resident in HBM; the mixed and
-FP64 solves alternate step by step (each timed on its own).  `value` is the
+FP64 solves alternate step by step (each timed on its own).  Every solve call
+returns only after an event synchronisation on its stream and the copies of
+its report, so each timed step is bracketed by device synchronisation.  No
+hipDeviceSynchronize is issued between the warm-up and the timed steps: on
+this runtime the launch after one takes about 6 ms longer.  `value` is the
 mixed-precision solve time in ms (lower is better).  Also reported: the FP64
 single-kernel CG time on the same matrix, the speedup, iterations, the FP64
 residual recomputed on the CPU, the per-tile precision histogram, the SpMV
@@ -36,6 +40,7 @@ The learned policy uses the MLP shipped with the package (policy_mlp.f32,
 trained on the GPU by millefeuille.policy.train_default).
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -235,8 +240,12 @@ def main():
     for _ in range(args.warmup):
         solve(policy)
         solve("fp64")
-    mf.synchronize()
+    # (each solve returns after its kernel and the report copies have
+    # completed, so the device is idle here)
     t_mixed, t_fp64 = [], []
+    # as timeit does: collect once, then no collector pauses inside the timed steps
+    gc.collect()
+    gc.disable()
     for _ in range(args.steps):
         t1 = time.perf_counter()
         rep = solve(policy)
@@ -245,6 +254,9 @@ def main():
         t2 = time.perf_counter()
         rep64 = solve("fp64")
         t_fp64.append(time.perf_counter() - t2)
+        log(f"step: mixed {1e3 * t_mixed[-1]:.2f} ms (kernel {rep['time_ms']:.2f}), "
+            f"fp64 {1e3 * t_fp64[-1]:.2f} ms (kernel {rep64['time_ms']:.2f})")
+    gc.enable()
     x64 = dx.get()[:n]
     mixed_ms = 1e3 * sum(t_mixed) / args.steps
     fp64_ms = 1e3 * sum(t_fp64) / args.steps
