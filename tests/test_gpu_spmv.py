@@ -208,17 +208,22 @@ def _arrow(n=3000, dense_rows=16, seed=9):
     return rp, c.astype(np.int32), rng.uniform(-1, 1, len(c))
 
 
-def test_spmv_window_walk_and_one_window_descriptors(mf, oracle):
-    """A tile-row larger than one window (188 tiles, 48000 entries) is walked
-    in several windows with its partial row sums carried; the band's
-    tile-rows take the one-window staging.  Both against the oracle, FP64
-    tiles and per-tile levels, at several wave schedules."""
-    rp, ci, va = _arrow()
-    n = len(rp) - 1
+@pytest.mark.parametrize("n", [600, 3000])
+def test_spmv_window_walk_pieces_and_one_window_descriptors(mf, oracle, n):
+    """Tile-row 0 of the arrow matrix exceeds one window (128 tiles or 4096
+    slots).  At n = 600 (9600 entries) one wave walks it in several windows,
+    carrying its partial row sums; at n = 3000 (48000 entries, above the
+    heavy-row bound 4*max(wave_nnz, 4096)) it is cut into one-window pieces
+    whose last finisher adds the row sums.  The band's tile-rows take the
+    one-window staging.  All against the oracle, FP64 tiles and per-tile
+    levels, at several wave schedules, and bit-identical on a repeat."""
+    rp, ci, va = _arrow(n=n)
     tm = _mk(mf, rp, ci, va)
     a = tm.arrays()
     tiles_per_row = np.diff(a["tile_ptr"])
-    assert tiles_per_row[0] > 128 and a["csr_offset"][tiles_per_row[0]] > 4096
+    row0_entries = a["csr_offset"][tiles_per_row[0]]
+    assert (tiles_per_row[0] > 128 or row0_entries > 4096) and tiles_per_row[1:].max() <= 4
+    assert (row0_entries > 16384) == (n == 3000)
     x = np.random.default_rng(3).uniform(-1, 1, n)
     dx, dy = tm.vector(x), tm.vector()
     dx32 = tm.vector(x.astype(np.float32), dtype=np.float32)
@@ -228,11 +233,14 @@ def test_spmv_window_walk_and_one_window_descriptors(mf, oracle):
     levels = np.random.default_rng(4).integers(0, 5, tm.tilenum).astype(np.uint8)
     ym, blow, bhigh = _dequant_reference(tm, levels, x)
     for wn in (256, 1024, 4096):
-        print(f"wave_nnz {wn}", flush=True)
+        print(f"n {n} wave_nnz {wn}", flush=True)
         tm.set_schedule(wn)
-        tm.set_precision(None, uniform=0)
         tm.spmv(dx, dy, mode=0)
-        assert np.all(np.abs(dy.get()[:n] - yr) <= 1e-14 * bound + 1e-300)
+        y0 = dy.get()[:n]
+        assert np.all(np.abs(y0 - yr) <= 1e-14 * bound + 1e-300)
+        tm.spmv(dx, dy, mode=0)
+        np.testing.assert_array_equal(dy.get()[:n], y0)
         tm.set_precision(levels)
         tm.spmv(dx, dy, mode=1, d_x32=dx32)
         assert np.all(np.abs(dy.get()[:n] - ym[:n]) <= 2e-6 * blow[:n] + 1e-14 * bhigh[:n] + 1e-300)
+        tm.set_precision(None, uniform=0)
