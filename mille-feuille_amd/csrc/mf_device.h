@@ -955,6 +955,15 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// A wave-uniform double moved to scalar registers, so that loop state which
+// every lane holds alike does not occupy vector registers.
+__device__ __forceinline__ double uniform_double(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // Deterministic sum over the block: wave sums, then waves in index order.
 // `lds` must hold blockDim.x/64 doubles.  Every thread gets the result.
 __device__ __forceinline__ double block_sum(double v, double *lds) {
@@ -965,7 +974,7 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
   __syncthreads();
   double s = 0.0;
   for (int i = 0; i < nw; ++i) s += lds[i];
-  return s;
+  return uniform_double(s);  // every lane summed the same values in the same order
 }
 
 // Grid synchronisation for a co-resident grid.
@@ -1061,15 +1070,6 @@ __device__ __forceinline__ void grid_sync(GridSync *gs, unsigned &epoch, int *ld
 // never reused while a slower block may still read it (two sync points
 // always separate two uses of one slot).
 constexpr int kMaxGridSum = 4;
-
-// A wave-uniform double moved to scalar registers, so that loop state which
-// every lane holds alike does not occupy vector registers.
-__device__ __forceinline__ double uniform_double(double v) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
 
 template <int NV>
 __device__ __forceinline__ void grid_sum(double (&v)[NV], double *partials, GridSync *gs,
