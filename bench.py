@@ -198,6 +198,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=0,
                     help="CPU baseline CG iterations (0: run it to the tolerance)")
     ap.add_argument("--spmv-runs", type=int, default=50)
+    ap.add_argument("--skip-cpu-baseline", action="store_true",
+                    help="profiling runs: leave out the CPU baseline (cpu_baseline is then null)")
     ap.add_argument("--wave-nnz", type=int, default=1024,
                     help="stand-alone SpMV entries per wave (0: pick with the autotuner)")
     ap.add_argument("--workload", default="lap3d128",
@@ -328,24 +330,10 @@ def main():
 
     # --- CPU baseline: the oracle's FP64 CG (1 thread) to the same tolerance,
     # or a bounded number of iterations scaled to the GPU's count ----------
-    tc = time.perf_counter()
-    if args.cpu_iters > 0:
-        _, it_cpu, rel_c = oracle.cg(rp, ci, va, b, tol=1e-30, maxiter=args.cpu_iters)
-    else:
-        _, it_cpu, rel_c = oracle.cg(rp, ci, va, b, tol=1e-10, maxiter=20000)
-    cpu_s = time.perf_counter() - tc
-    per_it_ms = cpu_s * 1e3 / max(it_cpu, 1)
-    cpu_ms = cpu_s * 1e3 if args.cpu_iters <= 0 else per_it_ms * rep64["iterations"]
-    ot = oracle.tile_create(n, n, rp, ci, va)
-    tcs = time.perf_counter()
-    oracle.blockspmv_tiled(ot, xs, nthreads=1)
-    cpu_spmv_ms = (time.perf_counter() - tcs) * 1e3
-    ncpu = len(os.sched_getaffinity(0))
-    tcs = time.perf_counter()
-    oracle.blockspmv_tiled(ot, xs, nthreads=ncpu)
-    cpu_spmv_mt_ms = (time.perf_counter() - tcs) * 1e3
-    log(f"cpu: CG {it_cpu} iterations in {cpu_s:.2f} s ({per_it_ms:.1f} ms/iteration, 1 thread), "
-        f"rel {rel_c:.2e}; tiled spmv {cpu_spmv_ms:.1f} ms (1 thread), {cpu_spmv_mt_ms:.1f} ms ({ncpu} threads)")
+    cpu_baseline = None
+    if not args.skip_cpu_baseline:
+        cpu_baseline = _cpu_baseline(args, rp, ci, va, b, xs, n, rep64)
+
 
     out = {
         "metric": "time-to-1e-10 residual (FP64-checked): learned/mixed vs FP64 CG; SpMV HBM GB/s",
@@ -375,18 +363,73 @@ def main():
                  "kernel": tm.spmv_kernel_name(), "copy_GBps_measured": round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "cpu_baseline": {"value": round(cpu_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
-                         "sample": (f"oracle FP64 CG (textbook, CSR SpMV, 1 thread) on this workload: "
-                                    f"{it_cpu} iterations in {cpu_s:.2f} s"
-                                    + ("" if args.cpu_iters <= 0 else
-                                       f", scaled to the GPU FP64 iteration count {rep64['iterations']}")
-                                    + f"; oracle tiled SpMV (blockspmv_cpu loop with Tile_csr_Col): "
-                                      f"{cpu_spmv_ms:.1f} ms on 1 thread, {cpu_spmv_mt_ms:.1f} ms on "
-                                      f"{ncpu} threads")},
+        "cpu_baseline": cpu_baseline,
     }
     if not (rel_cpu <= 1e-10 and rel64 <= 1e-10):
         out["error"] = "FP64-checked residual above 1e-10"
     print(json.dumps(out), flush=True)
+    _release(timer, dxs, dys, db, dx, tm)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_baseline(args, rp, ci, va, b, xs, n, rep64):
+    """The oracle's FP64 CG (textbook, CSR SpMV, 1 thread) to the tolerance on
+    this workload (or a bounded number of iterations scaled to the GPU's
+    count), and the oracle's tiled SpMV (the blockspmv_cpu tile loop with
+    Tile_csr_Col) on 1 thread and on every host core this process may use."""
+    import oracle
+    tc = time.perf_counter()
+    if args.cpu_iters > 0:
+        _, it_cpu, rel_c = oracle.cg(rp, ci, va, b, tol=1e-30, maxiter=args.cpu_iters)
+    else:
+        _, it_cpu, rel_c = oracle.cg(rp, ci, va, b, tol=1e-10, maxiter=20000)
+    cpu_s = time.perf_counter() - tc
+    per_it_ms = cpu_s * 1e3 / max(it_cpu, 1)
+    cpu_ms = cpu_s * 1e3 if args.cpu_iters <= 0 else per_it_ms * rep64["iterations"]
+    ot = oracle.tile_create(n, n, rp, ci, va)
+    ncpu = len(os.sched_getaffinity(0))
+    spmv = {}
+    for nt in (1, ncpu):
+        oracle.blockspmv_tiled(ot, xs, nthreads=nt)  # first touch
+        reps = []
+        for _ in range(5):
+            tcs = time.perf_counter()
+            oracle.blockspmv_tiled(ot, xs, nthreads=nt)
+            reps.append((time.perf_counter() - tcs) * 1e3)
+        spmv[nt] = float(np.median(reps))
+    cpu = _cpu_model()
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PLACES", "OMP_PROC_BIND")}
+    log(f"cpu ({cpu}, {ncpu} usable cores, {env}): CG {it_cpu} iterations in {cpu_s:.2f} s "
+        f"({per_it_ms:.1f} ms/iteration, 1 thread), rel {rel_c:.2e}; tiled spmv "
+        f"{spmv[1]:.1f} ms (1 thread), {spmv[ncpu]:.1f} ms ({ncpu} threads)")
+    return {"value": round(cpu_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "cpu_model": cpu, "usable_cores": ncpu, "omp_env": env,
+            "sample": (f"oracle FP64 CG (textbook, CSR SpMV, 1 thread) on this workload: "
+                       f"{it_cpu} iterations in {cpu_s:.2f} s, rel {rel_c:.2e}"
+                       + ("" if args.cpu_iters <= 0 else
+                          f", scaled to the GPU FP64 iteration count {rep64['iterations']}")
+                       + f"; oracle tiled SpMV (blockspmv_cpu loop with Tile_csr_Col), median of 5: "
+                         f"{spmv[1]:.1f} ms on 1 thread, {spmv[ncpu]:.1f} ms on {ncpu} threads"),
+            "spmv_ms_1thread": round(spmv[1], 2), "spmv_ms_all_threads": round(spmv[ncpu], 2)}
+
+
+def _release(*objs):
+    """Frees device objects explicitly, in order, while the runtime is fully
+    up (nothing is left for interpreter shutdown), then waits for the device."""
+    import millefeuille as mf
+    for o in objs:
+        o.free()
+    mf.synchronize()
 
 
 if __name__ == "__main__":

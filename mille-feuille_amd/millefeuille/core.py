@@ -7,6 +7,7 @@ entry points) with numpy arrays for host data and DeviceArray for HBM data.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -82,6 +83,10 @@ class TiledMatrix:
             self.h = None
 
     def __del__(self):
+        # nothing is released through HIP during interpreter shutdown (the
+        # runtime's exit handlers may have run; process exit frees it all)
+        if sys.is_finalizing():
+            return
         try:
             self.free()
         except Exception:

@@ -26,7 +26,9 @@ import numpy as np
 
 
 def _csr_from_coo(n, rows, cols, vals):
-    order = np.lexsort((cols, rows))
+    # row-major, ascending columns (one int64 key; the same order as
+    # np.lexsort((cols, rows)), several times faster at 60M entries)
+    order = np.argsort(rows.astype(np.int64) * n + cols, kind="stable")
     rows = rows[order]
     cols = cols[order].astype(np.int32)
     vals = vals[order].astype(np.float64)
@@ -148,6 +150,19 @@ def knn_laplacian(n, k=8, seed=4, shift=1e-2):
     return _csr_from_coo(n, rows, cols, vals)
 
 
+def _weighted_choice(rng, size, p):
+    """rng.choice(len(p), size=size, p=p) with the same draws and result
+    (cdf of p, one uniform per sample, right-sided search), the search done
+    on the sorted uniforms so that it walks the cdf in order."""
+    cdf = p.cumsum()
+    cdf /= cdf[-1]
+    u = rng.random(size)
+    order = np.argsort(u)
+    idx = np.empty(size, np.int64)
+    idx[order] = cdf.searchsorted(u[order], side="right")
+    return idx
+
+
 def powerlaw_laplacian(n, target_nnz, exponent=2.5, seed=5, shift=1e-3):
     """Chung-Lu random graph with expected degrees w_i ~ i^(-1/(exponent-1)),
     scaled so that the Laplacian has about target_nnz entries; L = D - A +
@@ -159,8 +174,8 @@ def powerlaw_laplacian(n, target_nnz, exponent=2.5, seed=5, shift=1e-3):
     p = w / w.sum()
     m_edges = max(1, (target_nnz - n) // 2)
     draw = int(m_edges * 1.15) + 16
-    a = rng.choice(n, size=draw, p=p)
-    b = rng.choice(n, size=draw, p=p)
+    a = _weighted_choice(rng, draw, p)
+    b = _weighted_choice(rng, draw, p)
     # random relabelling so hubs are spread over the tile rows
     perm = rng.permutation(n)
     a = perm[a]

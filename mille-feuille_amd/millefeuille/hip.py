@@ -8,6 +8,7 @@ device count.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -93,6 +94,11 @@ class DeviceArray:
             self.ptr = None
 
     def __del__(self):
+        # No HIP calls while the interpreter shuts down: the runtime's own
+        # exit handlers (and a profiler's) may already have run, and the
+        # process teardown releases device memory anyway.
+        if sys.is_finalizing():
+            return
         try:
             self.free()
         except Exception:
@@ -117,9 +123,16 @@ class Timer:
         _ok("hipEventElapsedTime", _hip.hipEventElapsedTime(C.byref(ms), self.e0, self.e1))
         return ms.value
 
-    def __del__(self):
-        try:
+    def free(self):
+        if self.e0:
             _hip.hipEventDestroy(self.e0)
             _hip.hipEventDestroy(self.e1)
+            self.e0 = self.e1 = None
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.free()
         except Exception:
             pass

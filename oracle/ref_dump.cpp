@@ -13,6 +13,11 @@
 // cg_solve_sync (src/main-cg.cu:379-424: Tile_create, then blockspmv_cpu).
 // blockspmv_cpu is called with a non-trivial x so that its (nibble-decoded)
 // product is a fixture too.
+// The reference's tile load balancer (src/main-cg.cu:596-769) and its
+// shared-queue offsets (src/main-cg-mixed.cu:2681-2709) are plain host C
+// inside .cu files; ref_extract.sh copies those line ranges into _ref/ at
+// build time and ref_balance() below compiles them as they are.
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -29,6 +34,35 @@ static void dump(const std::string &dir, const char *name, const void *p,
   FILE *f = std::fopen(path.c_str(), "wb");
   if (!f) { std::perror(path.c_str()); std::exit(2); }
   if (bytes) std::fwrite(p, 1, bytes, f);
+  std::fclose(f);
+}
+
+// Runs the reference's balancer and shared-queue offset loops on a tiled
+// matrix and writes their outputs; `warps` (the reference's `index`) stays
+// -1 when the balancer gives up (it returns when index > 6912).
+static void ref_balance(Tile_matrix *matrix, int nnzR, const std::string &out) {
+  int tilem = matrix->tilem;
+  int tilenum = matrix->tilenum;
+  MAT_PTR_TYPE *tile_ptr = matrix->tile_ptr;
+  int warps = -1;
+  std::string meta = out + "/balance_meta.txt";
+  FILE *f = std::fopen(meta.c_str(), "w");
+  std::fprintf(f, "warps %d\n", warps);
+  std::fclose(f);
+  {
+#include "ref_balance.inc"
+#include "ref_shared.inc"
+    warps = index;
+    dump(out, "balance_tile_ptr_new", balance_tile_ptr_new, sizeof(int) * (size_t)(index + 1));
+    dump(out, "index_each_block_new", index_each_block_new, sizeof(int) * (size_t)(tilenum + 1));
+    dump(out, "row_each_block_new", row_each_block_new, sizeof(int) * (size_t)(tilenum + 1));
+    dump(out, "non_each_block_new", non_each_block_new, sizeof(int) * (size_t)(tilenum + 1));
+    dump(out, "non_each_block_offset", non_each_block_offset, sizeof(int) * (size_t)(tilenum + 1));
+    dump(out, "balance_tile_ptr_shared_end", balance_tile_ptr_shared_end,
+         sizeof(int) * (size_t)(index + 1));
+  }
+  f = std::fopen(meta.c_str(), "w");
+  std::fprintf(f, "warps %d\n", warps);
   std::fclose(f);
 }
 
@@ -87,6 +121,8 @@ int main(int argc, char **argv) {
   dump(out, "blkcoostylerowidx_colstop", ce, sizeof(int) * (size_t)rbb);
   dump(out, "spmv_x", x, sizeof(double) * (size_t)xn);
   dump(out, "spmv_y_nibble", y, sizeof(double) * (size_t)rowA);
+
+  if (tn > 1) ref_balance(mat, nnzR, out);
 
   std::string meta = out + "/meta.txt";
   FILE *f = std::fopen(meta.c_str(), "w");
