@@ -449,6 +449,458 @@ __device__ __forceinline__ void chunk_consume(WaveLds &L, const Chunk &ch, bool 
   atomicAdd(&L.acc[key], run);
 }
 
+// ---------------------------------------------------------------------------
+// Level-grouped windows (per-tile levels).
+//
+// A window's chunks are streamed grouped by their tile's level.  At staging,
+// every tile's chunks get consecutive positions in its level's group (FP64,
+// then FP32, FP16, INT8, and last SKIP) from fixed-order wave prefix sums of
+// the per-level chunk counts, so the grouping, and with it the order of the
+// row accumulations, is the same on every run.  L.ctile[pos] then holds the
+// tile of the chunk at position pos, and L.tend[j] packs the end of tile j's
+// entries (bits 0-12) with the offset from the tile's first position to its
+// first chunk (bits 13-23, biased by 1024): chunk = pos + offset.  Each group
+// is streamed by a loop compiled for its level: payload loads of the level's
+// width (INT8 4 B, FP16 8 B, FP32 16 B, FP64 32 B per chunk), 4-byte x32
+// gathers below FP64, and a consume without level selection.  The SKIP group
+// is streamed only by an all-FP64 pass over every position.
+//
+// The functions take any LDS layout L with members tcol, tscale, tend, trow,
+// ctile and acc (WaveLds for streamed windows, ResLds for the solvers'
+// resident windows).
+// ---------------------------------------------------------------------------
+struct LvGroups {
+  // positions: FP64 [0,b1), FP32 [b1,b2), FP16 [b2,b3), INT8 [b3,e), SKIP [e,eall)
+  int b1, b2, b3, e, eall;
+};
+
+template <int LV>
+struct LvChunk {
+  uint32_t rc4;
+  int vmask, tcol, ro;
+  float sc;
+  u32x4_t a, b;           // payload words (b: FP64 only)
+  double d0, d1, d2, d3;  // x of FP64 tiles multiplying FP64 x
+  float f0, f1, f2, f3;   // x32 of the other cases
+};
+
+// Inclusive prefix sums of three values over the wave's lanes.
+__device__ __forceinline__ void wave_scan3(uint32_t &v0, uint32_t &v1, uint32_t &v2, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t u0 = __shfl_up(v0, d), u1 = __shfl_up(v1, d), u2 = __shfl_up(v2, d);
+    if (lane >= d) {
+      v0 += u0;
+      v1 += u1;
+      v2 += u2;
+    }
+  }
+}
+
+// Per-level chunk counts packed in three words: FP64 | FP32 << 16,
+// FP16 | INT8 << 16, SKIP (a window holds at most 1024 chunks).
+struct LvCount {
+  uint32_t lo, hi, sk;
+  __device__ __forceinline__ LvCount operator+(const LvCount &o) const {
+    return LvCount{lo + o.lo, hi + o.hi, sk + o.sk};
+  }
+  __device__ __forceinline__ LvCount operator-(const LvCount &o) const {
+    return LvCount{lo - o.lo, hi - o.hi, sk - o.sk};
+  }
+  __device__ __forceinline__ int field(int lvl) const {
+    if (lvl >= MF_PREC_SKIP) return (int)sk;
+    const uint32_t w = lvl < 2 ? lo : hi;
+    return (int)((lvl & 1) ? w >> 16 : w & 0xFFFFu);
+  }
+};
+
+__device__ __forceinline__ LvCount lv_pack(int lvl, int n) {
+  return LvCount{lvl == MF_PREC_FP64 ? (uint32_t)n : lvl == MF_PREC_FP32 ? (uint32_t)n << 16 : 0u,
+                 lvl == MF_PREC_FP16 ? (uint32_t)n : lvl == MF_PREC_INT8 ? (uint32_t)n << 16 : 0u,
+                 lvl >= MF_PREC_SKIP ? (uint32_t)n : 0u};
+}
+
+// Inclusive wave scan of a count; tot receives lane 63's value.
+__device__ __forceinline__ LvCount lv_scan(LvCount v, int lane, LvCount &tot) {
+  wave_scan3(v.lo, v.hi, v.sk, lane);
+  tot = LvCount{(uint32_t)__builtin_amdgcn_readlane(v.lo, 63), (uint32_t)__builtin_amdgcn_readlane(v.hi, 63),
+                (uint32_t)__builtin_amdgcn_readlane(v.sk, 63)};
+  return v;
+}
+
+__device__ __forceinline__ LvGroups lv_groups(const LvCount &T) {
+  LvGroups g;
+  g.b1 = (int)(T.lo & 0xFFFFu);
+  g.b2 = g.b1 + (int)(T.lo >> 16);
+  g.b3 = g.b2 + (int)(T.hi & 0xFFFFu);
+  g.e = g.b3 + (int)(T.hi >> 16);
+  g.eall = g.e + (int)T.sk;
+  return g;
+}
+
+__device__ __forceinline__ int lv_base(const LvGroups &g, int l) {
+  return l == MF_PREC_FP64 ? 0 : l == MF_PREC_FP32 ? g.b1 : l == MF_PREC_FP16 ? g.b2
+       : l == MF_PREC_INT8 ? g.b3 : g.e;
+}
+
+// Tile j of the window into LDS for the level-grouped stream: o, e: its
+// payload begin and end relative to the window (multiples of 4); pos: its
+// first position in its level's group.
+template <bool COUNT, class LDS>
+__device__ __forceinline__ void stage_tile_lv(LDS &L, int j, int o, int e, int col, int trow, int nz,
+                                              int lvl, float sc, int pos, PrecCount *cnt) {
+  if (COUNT) cnt->add(lvl);
+  L.tcol[j] = col << 4;
+  L.trow[j] = (uint8_t)trow;
+  L.tscale[j] = sc;
+  L.tend[j] = (o + nz + 1) | (((o >> 2) - pos + 1024) << 13);
+  for (int q = 0; q < ((e - o) >> 2); ++q) L.ctile[pos + q] = (uint8_t)j;
+}
+
+// Stages the window's tiles lane (a) and lane + 64 (b) of nt, grouped by
+// level; returns the group bounds (wave-uniform).  o*, e*: payload begin and
+// end relative to the window; r*: tile-row relative to the window.
+template <bool COUNT>
+__device__ __forceinline__ LvGroups stage_levels(WaveLds &L, int lane, int nt, int oa, int ea,
+                                                 int ca, int ra, int na, int la, float sa, int ob,
+                                                 int eb, int cb, int rb, int nb, int lb, float sb,
+                                                 PrecCount *cnt) {
+  const bool va = lane < nt, vb = lane + 64 < nt;
+  const LvCount ca_ = lv_pack(la, va ? (ea - oa) >> 2 : 0);
+  const LvCount cb_ = lv_pack(lb, vb ? (eb - ob) >> 2 : 0);
+  LvCount ta, tb;
+  const LvCount ia = lv_scan(ca_, lane, ta);
+  const LvCount ib = lv_scan(cb_, lane, tb);
+  const LvGroups g = lv_groups(ta + tb);
+  if (va) stage_tile_lv<COUNT>(L, lane, oa, ea, ca, ra, na, la, sa, lv_base(g, la) + (ia - ca_).field(la), cnt);
+  if (vb)
+    stage_tile_lv<COUNT>(L, lane + 64, ob, eb, cb, rb, nb, lb, sb,
+                         lv_base(g, lb) + (ib - cb_ + ta).field(lb), cnt);
+  return g;
+}
+
+// Element at a 32-bit byte offset from a wave-uniform base: the load takes
+// the base in scalar registers and one 32-bit offset register per lane (no
+// 64-bit address arithmetic per lane).  Vector offsets stay below 2^32 bytes
+// because handles are limited to 2^28 rows (mf_tile_create).
+template <class T>
+__device__ __forceinline__ T ld_at(const void *base, uint32_t byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+// Payload streams of a window: the level arrays from the window's first slot.
+struct WinPay {
+  const uint8_t *rc;
+  const void *v64, *v32, *v16, *v8;
+};
+
+__device__ __forceinline__ WinPay win_pay(const TileView &A, int Z0) {
+  return WinPay{A.rc + Z0, A.v64 + Z0, A.v32 + Z0, A.v16 + Z0, A.v8 + Z0};
+}
+
+// Chunk at group position pos (< end; a lane past the end repeats the last
+// chunk with no valid slot, so every slice issues the same loads): LDS
+// lookups, then the (row,col) bytes and the payload at the level's width.
+template <int LV, class LDS>
+__device__ __forceinline__ void lv_load(const WinPay &W, const LDS &L, int pos, int end,
+                                        LvChunk<LV> &s) {
+  const bool valid = pos < end;
+  const int p = valid ? pos : end - 1;
+  const int j = L.ctile[p];
+  const int te = L.tend[j];
+  const int c = p + (te >> 13) - 1024;
+  const int live = (te & 8191) - 4 * c;
+  s.vmask = valid ? (live >= 4 ? 15 : (1 << live) - 1) : 0;
+  s.tcol = L.tcol[j];
+  s.ro = L.trow[j] * 16;
+  s.sc = (LV == MF_PREC_FP16 || LV == MF_PREC_INT8) ? L.tscale[j] : 1.0f;
+  const uint32_t c4 = 4u * (uint32_t)c;  // first slot of the chunk in the window
+  s.rc4 = ld_at<uint32_t>(W.rc, c4);
+  if (LV == MF_PREC_FP64) {
+    s.a = ld_at<u32x4_t>(W.v64, 8u * c4);
+    s.b = ld_at<u32x4_t>(W.v64, 8u * c4 + 16u);
+  } else if (LV == MF_PREC_FP32) {
+    s.a = ld_at<u32x4_t>(W.v32, 4u * c4);
+  } else if (LV == MF_PREC_FP16) {
+    const uint2 h = ld_at<uint2>(W.v16, 2u * c4);
+    s.a.x = h.x;
+    s.a.y = h.y;
+  } else {
+    s.a.x = ld_at<uint32_t>(W.v8, c4);
+  }
+}
+
+// x at the chunk's four columns: FP64 x for FP64 tiles unless X32, else x32.
+template <int LV, bool X32>
+__device__ __forceinline__ void lv_gather(const double *__restrict__ x, const float *__restrict__ x32,
+                                          LvChunk<LV> &s) {
+  const uint32_t t = (uint32_t)s.tcol;
+  const uint32_t x0 = t + (s.rc4 & 15), x1 = t + ((s.rc4 >> 8) & 15);
+  const uint32_t x2 = t + ((s.rc4 >> 16) & 15), x3 = t + ((s.rc4 >> 24) & 15);
+  if (LV == MF_PREC_FP64 && !X32) {
+    s.d0 = ld_at<double>(x, 8u * x0);
+    s.d1 = ld_at<double>(x, 8u * x1);
+    s.d2 = ld_at<double>(x, 8u * x2);
+    s.d3 = ld_at<double>(x, 8u * x3);
+  } else {
+    s.f0 = ld_at<float>(x32, 4u * x0);
+    s.f1 = ld_at<float>(x32, 4u * x1);
+    s.f2 = ld_at<float>(x32, 4u * x2);
+    s.f3 = ld_at<float>(x32, 4u * x3);
+  }
+}
+
+template <int LV, bool X32>
+__device__ __forceinline__ double lv_product(const LvChunk<LV> &s, int u) {
+  const float xf = u == 0 ? s.f0 : u == 1 ? s.f1 : u == 2 ? s.f2 : s.f3;
+  if (LV == MF_PREC_FP64) {
+    const uint32_t lo = u == 0 ? s.a.x : u == 1 ? s.a.z : u == 2 ? s.b.x : s.b.z;
+    const uint32_t hi = u == 0 ? s.a.y : u == 1 ? s.a.w : u == 2 ? s.b.y : s.b.w;
+    const double a = __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    const double xd = u == 0 ? s.d0 : u == 1 ? s.d1 : u == 2 ? s.d2 : s.d3;
+    return a * (X32 ? (double)xf : xd);
+  } else if (LV == MF_PREC_FP32) {
+    const uint32_t w = u == 0 ? s.a.x : u == 1 ? s.a.y : u == 2 ? s.a.z : s.a.w;
+    return (double)(__builtin_bit_cast(float, w) * xf);
+  } else if (LV == MF_PREC_FP16) {
+    const uint32_t w = u < 2 ? s.a.x : s.a.y;
+    return (double)(half_bits_to_float((uint16_t)(w >> (16 * (u & 1)))) * xf * s.sc);
+  } else {
+    return (double)((float)(int8_t)(s.a.x >> (8 * u)) * xf * s.sc);
+  }
+}
+
+// Adds four slot products into row accumulators k0..k3 (see chunk_accumulate).
+template <class LDS>
+__device__ __forceinline__ void lv_accumulate(LDS &L, int k0, int k1, int k2, int k3, double p0,
+                                              double p1, double p2, double p3) {
+  const double s1 = p1 + (k1 == k0 ? p0 : 0.0);
+  const double s2 = p2 + (k2 == k1 ? s1 : 0.0);
+  const double s3 = p3 + (k3 == k2 ? s2 : 0.0);
+  if (k1 != k0) atomicAdd(&L.acc[k0], p0);
+  if (k2 != k1) atomicAdd(&L.acc[k1], s1);
+  if (k3 != k2) atomicAdd(&L.acc[k2], s2);
+  atomicAdd(&L.acc[k3], s3);
+}
+
+// Products of the chunk's valid slots into the window's row accumulators.
+// Every slot's product is computed and then selected (no branch per slot, so
+// no wait inside a branch).
+template <int LV, bool X32, class LDS>
+__device__ __forceinline__ void lv_consume(LDS &L, const LvChunk<LV> &s) {
+  if (!s.vmask) return;
+  double p[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    double v = lv_product<LV, X32>(s, u);
+    asm volatile("" : "+v"(v));
+    p[u] = (s.vmask >> u & 1) ? v : 0.0;
+  }
+  const int r0 = s.ro + ((s.rc4 >> 4) & 15), r1 = s.ro + ((s.rc4 >> 12) & 15);
+  const int r2 = s.ro + ((s.rc4 >> 20) & 15), r3 = s.ro + ((s.rc4 >> 28) & 15);
+  lv_accumulate(L, r0, r1, r2, r3, p[0], p[1], p[2], p[3]);
+}
+
+// Group positions [b, e) in slices of 64 chunks, D - 1 slices ahead: while
+// slice k is consumed, slice k+1's gathers and the loads of slices k+2 ..
+// k+D-1 are in flight.  Each step issues slice k+D-1's loads before slice
+// k+1's gathers, so waiting for slice k+1's (row,col) bytes (loaded D - 2
+// steps earlier) never waits for slice k's gathers (loads retire in issue
+// order), and a step waits on no load younger than one step.  The loop body
+// issues the same loads on every trip (slices past the group repeat its last
+// chunk with no valid slot), so the number of loads in flight at every wait
+// is fixed.  FP64 chunks with FP64 x hold the most registers and run three
+// slices deep, the others four.
+template <int LV, bool X32>
+struct LvDepth {
+  static constexpr int D = (LV == MF_PREC_FP64 && !X32) ? 3 : 4;
+};
+
+template <int LV, bool X32, class LDS>
+__device__ __forceinline__ void level_group(const TileView &A, LDS &L, int Z0, int b, int e,
+                                            const double *__restrict__ x,
+                                            const float *__restrict__ x32, int lane) {
+  constexpr int D = LvDepth<LV, X32>::D;
+  const int nsl = (e - b + 63) >> 6;
+  const int nslD = (nsl + D - 1) / D * D;  // whole trips of the unrolled body
+  const int p = b + lane;
+  const WinPay W = win_pay(A, Z0);
+  // The slices rotate through c[] by unrolling D steps (a register copy of a
+  // chunk whose loads are in flight would wait for them).
+  LvChunk<LV> c[D];
+  // The scheduling barriers keep the issue order of the loads as written
+  // (the scheduler would otherwise move a slice's loads, which wait on LDS
+  // lookups, behind the next gathers).
+#pragma unroll
+  for (int i = 0; i < D - 1; ++i) {
+    lv_load<LV>(W, L, p + 64 * i, e, c[i]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  lv_gather<LV, X32>(x, x32, c[0]);
+  for (int k = 0; k < nslD; k += D) {
+    const int q = p + 64 * (k + D - 1);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      lv_load<LV>(W, L, q + 64 * i, e, c[(i + D - 1) % D]);
+      __builtin_amdgcn_sched_barrier(0);
+      lv_gather<LV, X32>(x, x32, c[(i + 1) % D]);
+      __builtin_amdgcn_sched_barrier(0);
+      lv_consume<LV, X32>(L, c[i]);
+    }
+  }
+}
+
+// The staged window's groups.  all64: every position (SKIP included) at
+// FP64 with FP64 x; otherwise one level at a time, x32only making FP64 tiles
+// multiply x32 (in FP64 arithmetic).
+template <class LDS>
+__device__ __forceinline__ void level_groups(const TileView &A, LDS &L, int Z0, const LvGroups &g,
+                                             const double *__restrict__ x,
+                                             const float *__restrict__ x32, bool all64,
+                                             bool x32only, int lane) {
+  if (all64) {
+    if (g.eall > 0) level_group<MF_PREC_FP64, false>(A, L, Z0, 0, g.eall, x, x32, lane);
+    return;
+  }
+  if (g.b1 > 0) {
+    if (x32only)
+      level_group<MF_PREC_FP64, true>(A, L, Z0, 0, g.b1, x, x32, lane);
+    else
+      level_group<MF_PREC_FP64, false>(A, L, Z0, 0, g.b1, x, x32, lane);
+  }
+  if (g.b2 > g.b1) level_group<MF_PREC_FP32, true>(A, L, Z0, g.b1, g.b2, x, x32, lane);
+  if (g.b3 > g.b2) level_group<MF_PREC_FP16, true>(A, L, Z0, g.b2, g.b3, x, x32, lane);
+  if (g.e > g.b3) level_group<MF_PREC_INT8, true>(A, L, Z0, g.b3, g.e, x, x32, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Resident windows of the persistent solvers.
+//
+// A solver wave whose SpMV share is a tile-row range of at most kResRows
+// tile-rows, kResTiles tiles and kResChunks chunks (no heavy tile-row) keeps
+// that whole share staged in its LDS for the length of the solve: the tile
+// metadata is read from HBM once, and again only when the tile levels change
+// (a policy update or the FP64 fallback), when the level groups are rebuilt.
+// An SpMV is then only the payload stream of one window: no descriptor, no
+// metadata loads and no window cuts on the critical path.
+// ---------------------------------------------------------------------------
+constexpr int kResTiles = 256;
+constexpr int kResRows = 40;
+constexpr int kResChunks = 1024;
+
+struct ResLds {
+  int tcol[kResTiles];        // 16 * tile column
+  float tscale[kResTiles];    // FP16 / INT8 scale of the tile's current level
+  int tend[kResTiles];        // entry end | position offset (level groups)
+  uint8_t trow[kResTiles];    // tile-row relative to the wave's first
+  uint8_t ctile[kResChunks];  // tile of each group position
+  double acc[kResRows * 16];
+};
+
+// The wave's resident window: tile-rows [tr0, tr1), tiles [ta, ta + nt),
+// payload from slot Z0, its level groups and per-level tile counts (for the
+// precision histogram).
+struct ResWin {
+  int tr0, tr1, ta, nt, Z0;
+  LvGroups g;
+  unsigned n64, n32, n16, n8, nskip;
+};
+
+// Whether tile-rows [tr0, tr1) fit a resident window (host and device).
+__host__ __device__ __forceinline__ bool res_fits(int rows, int tiles, int chunks) {
+  return rows <= kResRows && tiles <= kResTiles && chunks <= kResChunks;
+}
+
+// Stages (or re-stages) the resident window of tile-rows [tr0, tr1) with the
+// tiles' current levels (FP64 for all when all64).  Tiles lane + 64 q,
+// q = 0..3, are taken in that order.
+template <int MODE>
+__device__ __forceinline__ void res_stage(const TileView &A, ResLds &L, ResWin &w, int tr0, int tr1,
+                                          const uint8_t *__restrict__ seglev, bool all64, int lane) {
+  const int ta = __builtin_amdgcn_readfirstlane(A.tile_ptr[tr0]);
+  const int nt = __builtin_amdgcn_readfirstlane(A.tile_ptr[tr1]) - ta;
+  const int Z0 = __builtin_amdgcn_readfirstlane(A.tile_off[ta]);
+  w.tr0 = tr0;
+  w.tr1 = tr1;
+  w.ta = ta;
+  w.nt = nt;
+  w.Z0 = Z0;
+  if (nt <= 0) {  // no tiles (a wave past the last tile-row): nothing to load
+    w.g = LvGroups{0, 0, 0, 0, 0};
+    w.n64 = w.n32 = w.n16 = w.n8 = w.nskip = 0;
+    return;
+  }
+  int o[4], e[4], col[4], row[4], nz[4], lv[4];
+  float sc[4];
+  LvCount c[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = lane + 64 * q;
+    const bool v = j < nt;
+    const int t = ta + (v ? j : nt - 1);  // lanes past the last tile reload it
+    o[q] = A.tile_off[t] - Z0;
+    e[q] = A.tile_off[t + 1] - Z0;
+    col[q] = A.tile_col[t];
+    row[q] = A.tile_row[t] - tr0;
+    nz[q] = A.tile_nz[t];
+    lv[q] = MODE == kAll64 || all64 ? MF_PREC_FP64
+          : MODE == kPerTile ? A.prec[t] : seglev[col[q]];
+    sc[q] = lv[q] == MF_PREC_FP16 ? A.s16[t] : A.s8[t];
+    c[q] = lv_pack(lv[q], v ? (e[q] - o[q]) >> 2 : 0);
+  }
+  LvCount tot{0u, 0u, 0u}, pre[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    LvCount tq;
+    pre[q] = lv_scan(c[q], lane, tq) - c[q] + tot;
+    tot = tot + tq;
+  }
+  w.g = lv_groups(tot);
+  unsigned n[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = lane + 64 * q;
+    if (j < nt)
+      stage_tile_lv<false>(L, j, o[q], e[q], col[q], row[q], nz[q], lv[q], sc[q],
+                           lv_base(w.g, lv[q]) + pre[q].field(lv[q]), nullptr);
+#pragma unroll
+    for (int l = 0; l < 5; ++l)
+      n[l] += (unsigned)__builtin_popcountll(__ballot(j < nt && lv[q] == l));
+  }
+  w.n64 = n[0];
+  w.n32 = n[1];
+  w.n16 = n[2];
+  w.n8 = n[3];
+  w.nskip = n[4];
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One SpMV of the resident window through epi(row, value).
+template <bool COUNT, typename Epi>
+__device__ __forceinline__ void res_spmv(const TileView &A, ResLds &L, const ResWin &w,
+                                         const double *__restrict__ x,
+                                         const float *__restrict__ x32, bool all64, bool x32only,
+                                         int lane, PrecCount *cnt, Epi epi) {
+  const int nr = (w.tr1 - w.tr0) * 16;
+  for (int i = lane; i < nr; i += 64) L.acc[i] = 0.0;
+  // (the counts are the wave's, alike in every lane: kept in scalar
+  // registers, and the histogram flush takes lane 0's)
+  if (COUNT && all64) {
+    cnt->c0 += (unsigned)w.nt;
+  } else if (COUNT) {
+    cnt->c0 += w.n64;
+    cnt->c1 += w.n32;
+    cnt->c2 += w.n16;
+    cnt->c3 += w.n8;
+    cnt->c4 += w.nskip;
+  }
+  __builtin_amdgcn_wave_barrier();
+  level_groups(A, L, w.Z0, w.g, x, x32, all64, x32only, lane);
+  __builtin_amdgcn_wave_barrier();
+  for (int i = lane; i < nr; i += 64) epi(16 * w.tr0 + i, L.acc[i]);
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Descriptor of tile-rows [tr0, tr1) from the tile pointers (callers without
 // a precomputed one).
 __device__ __forceinline__ WaveDesc wave_desc(const TileView &A, int tr0, int tr1) {
@@ -644,25 +1096,27 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
         qb = A.s8[ta + jb];
       }
     }
-    Chunk cx, cy;
-    chunk_load_rc(A, Z0, lane, nch, all64, cx);
     // Tile j's payload ends where tile j + 1's begins (lane + 1's offset;
     // for lane 63's first tile, lane 0's second), the last tile's at Z1.
     const int oa_next = __shfl(oa, (lane + 1) & 63);
     const int ob_next = __shfl(ob, (lane + 1) & 63);
     const int ob_first = __shfl(ob, 0);
-    if (ja < nt) {
-      const int e = ja + 1 < nt ? (lane < 63 ? oa_next : ob_first) : Z1;
-      stage_tile<MODE, COUNT>(L, ja, oa - Z0, e - Z0, ca, wa - tr0, na, la,
-                              la == MF_PREC_FP16 ? ha : qa, cnt);
+    const int ea = ja + 1 < nt ? (lane < 63 ? oa_next : ob_first) : Z1;
+    const int eb = jb + 1 < nt ? ob_next : Z1;
+    if constexpr (MODE != kAll64) {
+      const LvGroups g = stage_levels<COUNT>(L, lane, nt, oa - Z0, ea - Z0, ca, wa - tr0, na, la,
+                                             la == MF_PREC_FP16 ? ha : qa, ob - Z0, eb - Z0, cb,
+                                             wb - tr0, nb, lb, lb == MF_PREC_FP16 ? hb : qb, cnt);
+      __builtin_amdgcn_wave_barrier();
+      level_groups(A, L, Z0, g, x, x32, all64, x32only, lane);
+    } else {
+      Chunk cx, cy;
+      chunk_load_rc(A, Z0, lane, nch, all64, cx);
+      if (ja < nt) stage_tile<MODE, COUNT>(L, ja, oa - Z0, ea - Z0, ca, wa - tr0, na, la, 0.0f, cnt);
+      if (jb < nt) stage_tile<MODE, COUNT>(L, jb, ob - Z0, eb - Z0, cb, wb - tr0, nb, lb, 0.0f, cnt);
+      __builtin_amdgcn_wave_barrier();
+      if (nch > 0) window_slices<MODE, DEPTH, FLAT>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
     }
-    if (jb < nt) {
-      const int e = jb + 1 < nt ? ob_next : Z1;
-      stage_tile<MODE, COUNT>(L, jb, ob - Z0, e - Z0, cb, wb - tr0, nb, lb,
-                              lb == MF_PREC_FP16 ? hb : qb, cnt);
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (nch > 0) window_slices<MODE, DEPTH, FLAT>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
     __builtin_amdgcn_wave_barrier();
     for (int i = lane; i < (tr1 - tr0) * 16; i += 64) epi(tr0 * 16 + i, L.acc[i]);
     __builtin_amdgcn_wave_barrier();
@@ -720,23 +1174,28 @@ __device__ __forceinline__ void wave_spmv(const TileView &A, const WaveDesc &des
           sb = lb == MF_PREC_FP16 ? A.s16[ta + jb] : A.s8[ta + jb];
         }
       }
-      // The first slice's loads that need no metadata go out now.
-      Chunk cx, cy;
-      chunk_load_rc(A, Z0, lane, nch, all64, cx);
       // Metadata into LDS.  The payload end of tile j is the offset of tile
       // j + 1 (shuffled in uniform control flow: lane + 1's oa, for lane 63
       // lane 0's ob; lane + 1's ob, for lane 63 oc).
       const int na_next = __shfl(oa, (lane + 1) & 63);
       const int nb_next = __shfl(ob, (lane + 1) & 63);
       const int ob_first = __shfl(ob, 0);
-      if (ja < nt)
-        stage_tile<MODE, COUNT>(L, ja, oa - Z0, (lane < 63 ? na_next : ob_first) - Z0, ca,
-                                wa - ra, na, la, sa, cnt);
-      if (jb < nt)
-        stage_tile<MODE, COUNT>(L, jb, ob - Z0, (lane < 63 ? nb_next : oc) - Z0, cb, wb - ra, nb,
-                                lb, sb, cnt);
-      __builtin_amdgcn_wave_barrier();
-      window_slices<MODE, DEPTH, FLAT>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
+      const int ea = lane < 63 ? na_next : ob_first;
+      const int eb = lane < 63 ? nb_next : oc;
+      if constexpr (MODE != kAll64) {
+        const LvGroups g = stage_levels<COUNT>(L, lane, nt, oa - Z0, ea - Z0, ca, wa - ra, na, la, sa,
+                                               ob - Z0, eb - Z0, cb, wb - ra, nb, lb, sb, cnt);
+        __builtin_amdgcn_wave_barrier();
+        level_groups(A, L, Z0, g, x, x32, all64, x32only, lane);
+      } else {
+        // The first slice's loads (no metadata needed) go out first.
+        Chunk cx, cy;
+        chunk_load_rc(A, Z0, lane, nch, all64, cx);
+        if (ja < nt) stage_tile<MODE, COUNT>(L, ja, oa - Z0, ea - Z0, ca, wa - ra, na, la, sa, cnt);
+        if (jb < nt) stage_tile<MODE, COUNT>(L, jb, ob - Z0, eb - Z0, cb, wb - ra, nb, lb, sb, cnt);
+        __builtin_amdgcn_wave_barrier();
+        window_slices<MODE, DEPTH, FLAT>(A, L, Z0, nch, x, x32, all64, x32only, lane, cx, cy);
+      }
       __builtin_amdgcn_wave_barrier();
     }
     // Tile-rows [ra, R) are complete; tile-row R has started iff tile_ptr[R] < tb.
@@ -1001,7 +1460,6 @@ struct GridSync {
   unsigned pad0[63];
   unsigned abort;
   unsigned pad1[63];
-  double result[2][8];
   unsigned arrive[kMaxBlocks];
 };
 
@@ -1062,13 +1520,63 @@ __device__ __forceinline__ void grid_sync(GridSync *gs, unsigned &epoch, int *ld
   grid_arrive_wait(gs, epoch, [] {});
 }
 
+// Sum over the block of NV values per thread: fixed-order wave trees, then
+// the waves' sums in wave order.  `lds` must hold NV * blockDim.x/64
+// doubles.  Every thread gets the NV totals.
+template <int NV>
+__device__ __forceinline__ void block_sum_n(double (&v)[NV], double *lds) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) lds[i * nw + w] = v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int k = 0; k < nw; ++k) s += lds[i * nw + k];
+    v[i] = uniform_double(s);  // every lane summed the same values in the same order
+  }
+}
+
+// Barrier of the block's waves for LDS traffic only: it waits for the
+// wave's own LDS operations, not for its outstanding global stores (which a
+// full __syncthreads would drain).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// block_sum_n with LDS-only barriers (callers whose global stores need no
+// ordering against the sum).
+template <int NV>
+__device__ __forceinline__ void block_sum_lds(double (&v)[NV], double *lds) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  lds_barrier();
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) lds[i * nw + w] = v[i];
+  }
+  lds_barrier();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int k = 0; k < nw; ++k) s += lds[i * nw + k];
+    v[i] = uniform_double(s);
+  }
+}
+
 // Grid-wide deterministic sum of NV values per thread, also a full barrier.
-// Each block writes its block sums to partials[slot][i][block]; block 0,
-// once every arrival is in, adds them in block order with one wave (strided
-// over the lanes, then a fixed xor tree) and writes result[slot]; every block
-// then reads the same totals.  The two slots alternate so that storage is
-// never reused while a slower block may still read it (two sync points
-// always separate two uses of one slot).
+// Each block writes its NV block sums to partials[slot][block][i] and
+// arrives; once block 0 has seen every arrival and published the barrier,
+// every block reads all the partials and adds them in block order with the
+// same fixed-order tree, so every block gets the same totals without a
+// second broadcast.  The two slots alternate so that storage is never
+// reused while a slower block may still read it (two sync points always
+// separate two uses of one slot).  `lds` must hold NV * blockDim.x/64
+// doubles.
 constexpr int kMaxGridSum = 4;
 
 template <int NV>
@@ -1077,30 +1585,123 @@ __device__ __forceinline__ void grid_sum(double (&v)[NV], double *partials, Grid
   static_assert(NV <= kMaxGridSum, "grid_sum width");
   (void)lds_i;
   const int G = gridDim.x;
-  double bs[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) bs[i] = block_sum(v[i], lds);
+  block_sum_n<NV>(v, lds);
   double *buf = partials + (size_t)slot * kMaxGridSum * G;
-  double *res = gs->result[slot];
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) buf[i * G + blockIdx.x] = bs[i];
+    for (int i = 0; i < NV; ++i) buf[blockIdx.x * NV + i] = v[i];
   }
-  grid_arrive_wait(gs, epoch, [&] {
-    if (threadIdx.x < 64) {
-      const int lane = threadIdx.x;
+  grid_arrive_wait(gs, epoch, [] {});
+  double s[NV];
 #pragma unroll
-      for (int i = 0; i < NV; ++i) {
-        double s = 0.0;
-        for (int b = lane; b < G; b += 64) s += buf[i * G + b];
-        s = wave_sum(s);
-        if (lane == 0) res[i] = s;
-      }
-    }
-  });
+  for (int i = 0; i < NV; ++i) s[i] = 0.0;
+  for (int b = threadIdx.x; b < G; b += blockDim.x) {
+    double t[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = uniform_double(res[i]);
+    for (int i = 0; i < NV; ++i) t[i] = buf[b * NV + i];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s[i] += t[i];
+  }
+  block_sum_n<NV>(s, lds);
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = s[i];
   slot ^= 1;
+}
+
+// ---------------------------------------------------------------------------
+// Value-only grid sum with epoch-tagged mailboxes.
+//
+// Every 64-bit mailbox word carries the sync point's epoch in its high half
+// and 32 bits of a value in its low half (64-bit agent-scope atomic stores
+// and loads), so a reader that sees the epoch in every word of a mailbox
+// holds the whole value, with no memory fence, no drain of other stores and
+// no second round trip.  Each block posts its NV block sums in its mailbox;
+// block 0 reads every mailbox and adds them in block order with a fixed
+// tree, and posts the totals in the result mailbox, which every other
+// block's thread 0 polls.  This orders no other memory: it is used where the
+// next phase reads only the calling wave's own rows (the rows its SpMV
+// epilogue wrote), and a full barrier (grid_sync, grid_sum) precedes every
+// phase that reads other waves' data.  Waits are bounded as in wait_word.
+// ---------------------------------------------------------------------------
+constexpr int kMboxWords = 2 * kMaxGridSum;  // one 64-byte line per block
+
+template <int NV>
+__device__ __forceinline__ void mbox_post(unsigned long long *mb, unsigned epoch, const double (&v)[NV]) {
+  const unsigned long long e = (unsigned long long)epoch << 32;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v[i]);
+    __hip_atomic_store(mb + 2 * i, e | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(mb + 2 * i + 1, e | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <int NV>
+__device__ __forceinline__ bool mbox_take(const unsigned long long *mb, unsigned epoch, GridSync *gs,
+                                          double (&v)[NV]) {
+  unsigned long long w[2 * NV];
+  unsigned n = 0;
+  for (;;) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 2 * NV; ++k) {
+      w[k] = __hip_atomic_load(mb + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ok = ok && (unsigned)(w[k] >> 32) == epoch;
+    }
+    if (ok) break;
+    if (++n > kSpinLimit) {
+      __hip_atomic_store(&gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    if ((n & 1023u) == 0 && __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    v[i] = __builtin_bit_cast(double, ((w[2 * i + 1] & 0xffffffffull) << 32) | (w[2 * i] & 0xffffffffull));
+  return true;
+}
+
+// mbox: (gridDim.x + 1) * kMboxWords words, zero before the launch.
+template <int NV>
+__device__ __forceinline__ void grid_sum_values(double (&v)[NV], unsigned long long *mbox, GridSync *gs,
+                                                unsigned &epoch, double *lds) {
+  static_assert(NV <= kMaxGridSum, "grid_sum_values width");
+  __shared__ double bcast[kMaxGridSum];
+  const int G = gridDim.x;
+  block_sum_lds<NV>(v, lds);
+  ++epoch;
+  if (threadIdx.x == 0) mbox_post<NV>(mbox + (size_t)blockIdx.x * kMboxWords, epoch, v);
+  unsigned long long *res = mbox + (size_t)G * kMboxWords;
+  if (blockIdx.x == 0) {
+    double s[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s[i] = 0.0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x) {
+      double t[NV];
+      if (!mbox_take<NV>(mbox + (size_t)b * kMboxWords, epoch, gs, t)) break;
+#pragma unroll
+      for (int i = 0; i < NV; ++i) s[i] += t[i];
+    }
+    block_sum_lds<NV>(s, lds);
+    if (threadIdx.x == 0) mbox_post<NV>(res, epoch, s);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = s[i];
+  } else {
+    if (threadIdx.x == 0) {
+      double t[NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i) t[i] = 0.0;
+      mbox_take<NV>(res, epoch, gs, t);
+#pragma unroll
+      for (int i = 0; i < NV; ++i) bcast[i] = t[i];
+    }
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = uniform_double(bcast[i]);
+  }
+  lds_barrier();  // bcast and lds are free again
 }
 
 }  // namespace mf

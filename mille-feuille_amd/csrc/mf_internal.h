@@ -89,6 +89,10 @@ struct Schedule {
   // (the vector phases' row ownership).
   int nwaves = 0;
   int *wave_tr = nullptr;  // device, nwaves+1 tile-row bounds
+  // 1 for a wave whose SpMV share (its tile-rows, no heavy one, no pieces)
+  // fits one resident LDS window of the persistent solvers
+  int *wave_res = nullptr;  // device, nwaves
+  int all_resident = 0;     // every non-empty wave's share is resident
   // SpMV work of each wave: win_desc[wave_win[w] .. wave_win[w+1]), the
   // window descriptors of its tile-rows except heavy ones, then the pieces of
   // heavy tile-rows assigned to it.

@@ -23,6 +23,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <cstdio>
+#include <vector>
+#include <type_traits>
 
 #include "mf_device.h"
 
@@ -63,6 +66,7 @@ struct SolveArgs {
   const int *heavy_n;        // pieces of each heavy tile-row
   const int *heavy_base;     // first piece slot of each heavy tile-row
   double *partials;
+  unsigned long long *mbox;   // value-only grid sums: (blocks + 1) mailboxes
   GridSync *gs;
   double *stats;
   unsigned long long *hist;   // 5 counters
@@ -80,7 +84,29 @@ struct SolveArgs {
   float *seg2;                // per-segment ||v_seg||^2 of the multiplied vector
   uint8_t *prec_rw;           // tile levels written by the inference
   int nseg;
+#ifdef MF_PHASE_TRACE
+  unsigned long long *trace;  // per (iteration, phase, block) wall clock
+#endif
 };
+
+#ifdef MF_PHASE_TRACE
+// Profiling builds: thread 0 of every block records the wall clock (100 MHz)
+// at the phase boundaries of the first kTraceIters CG iterations.
+constexpr int kTraceIters = 64, kTracePhases = 5;
+#define MF_MARK(ph)                                                                         \
+  do {                                                                                     \
+    if (a.trace && trace_it < kTraceIters) {                                               \
+      __syncthreads();                                                                     \
+      if (threadIdx.x == 0)                                                                \
+        a.trace[((size_t)trace_it * kTracePhases + (ph)) * gridDim.x + blockIdx.x] =       \
+            wall_clock64();                                                                \
+    }                                                                                      \
+  } while (0)
+#else
+#define MF_MARK(ph) \
+  do {              \
+  } while (0)
+#endif
 
 // Reference rule (src/main-cg-mixed.cu:1326-1363): level from the absolute
 // change of a 16-entry segment of the search direction.
@@ -119,9 +145,12 @@ __device__ __forceinline__ InferCtx infer_ctx(const SolveArgs &a, double v2, dou
   return c;
 }
 
+// uniform: the counters are per wave (alike in every lane; lane 0's count).
 __device__ __forceinline__ void flush_hist(const PrecCount &c, unsigned long long *hist,
-                                           double *lds) {
-  double v0 = c.c0, v1 = c.c1, v2 = c.c2, v3 = c.c3, v4 = c.c4;
+                                           double *lds, bool uniform = false) {
+  const bool take = !uniform || (threadIdx.x & 63) == 0;
+  double v0 = take ? c.c0 : 0, v1 = take ? c.c1 : 0, v2 = take ? c.c2 : 0, v3 = take ? c.c3 : 0,
+         v4 = take ? c.c4 : 0;
   v0 = block_sum(v0, lds);
   v1 = block_sum(v1, lds);
   v2 = block_sum(v2, lds);
@@ -136,33 +165,69 @@ __device__ __forceinline__ void flush_hist(const PrecCount &c, unsigned long lon
   }
 }
 
+// A solver wave's SpMV state.  RES: every wave's share is a resident window
+// (ResLds, staged once); otherwise every wave streams its descriptors through
+// a WaveLds.  The host picks the kernel: the resident one when every wave's
+// share fits a window.
+template <bool RES>
+struct WaveCtx {
+  static constexpr bool res = RES;
+  typename std::conditional<RES, ResLds, WaveLds>::type *L;
+  ResWin win;
+};
+
 // SpMV of the calling wave's share: its descriptors (whole tile-rows) with
 // epi(row, value), then its pieces of heavy tile-rows (wave_spmv_piece).
-template <int MODE, bool COUNT, typename Epi>
+template <int MODE, bool COUNT, bool RES, typename Epi>
 __device__ __forceinline__ void solver_spmv(const SolveArgs &a, const double *__restrict__ x,
                                             const float *__restrict__ x32,
                                             const uint8_t *__restrict__ seglev, int force64,
-                                            WaveLds &L, int lane, PrecCount *cnt, Epi epi) {
+                                            WaveCtx<RES> &ctx, int lane, PrecCount *cnt, Epi epi) {
+  if constexpr (RES) {
+    const bool all64 = MODE == kAll64 || (force64 & 1);
+    res_spmv<COUNT>(a.A, *ctx.L, ctx.win, x, x32, all64, !all64 && (force64 & 2), lane, cnt, epi);
+    return;
+  }
+  WaveLds &L = *reinterpret_cast<WaveLds *>(ctx.L);
   const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const int d0 = __builtin_amdgcn_readfirstlane(a.wave_win[wid]);
   const int d1 = __builtin_amdgcn_readfirstlane(a.wave_win[wid + 1]);
   const HeavyView hv{a.heavy_part, a.heavy_cnt, a.heavy_n, a.heavy_base};
-  // per-tile levels consumed without branches on the slot data (measured
-  // faster in the solver loop too, despite a few more live registers)
   for (int d = d0; d < d1; ++d)
     wave_spmv_piece<MODE, COUNT, 3, false, MODE != kAll64>(a.A, a.win_desc[d], hv, x, x32, seglev,
                                                           force64, L, lane, cnt, epi);
+}
+
+// Dot products whose next phase reads only the wave's own rows.  With
+// resident windows a wave's SpMV epilogue writes exactly its own rows, so
+// the values-only grid sum suffices; streamed windows may hand a heavy
+// tile-row's epilogue to another wave, so they take the full barrier.
+template <bool RES, int NV>
+__device__ __forceinline__ void grid_sum_own(const SolveArgs &a, double (&v)[NV], unsigned &epoch,
+                                             int &slot, double *lds, int *lds_i) {
+  if constexpr (RES)
+    grid_sum_values<NV>(v, a.mbox, a.gs, epoch, lds);
+  else
+    grid_sum<NV>(v, a.partials, a.gs, epoch, slot, lds, lds_i);
+}
+
+// (Re-)stages a resident wave's window with the current tile levels.
+template <int MODE, bool RES>
+__device__ __forceinline__ void solver_stage(const SolveArgs &a, WaveCtx<RES> &ctx, int tr0, int tr1,
+                                             const uint8_t *__restrict__ seglev, int lane) {
+  if constexpr (RES) res_stage<MODE>(a.A, *ctx.L, ctx.win, tr0, tr1, seglev, false, lane);
 }
 
 // True residual of the wave's SpMV share (its tile-rows, and the heavy
 // tile-rows whose last piece it completes): r_new = b - A x (FP64), optionally
 // stored in FP64 (store) or FP32 (store32).  Returns the wave's share of
 // ||r_new||^2 (of the stored FP32 values when store32 is given).
+template <bool RES>
 __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int lane, double *store,
-                                                     WaveLds *wl,
+                                                     WaveCtx<RES> &ctx,
                                                      float *store32 = nullptr) {
   double acc = 0.0;
-  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, ctx, lane, nullptr,
     [&](const int i, const double ax) {
       double rv = a.b[i] - ax;
       if (store) store[i] = rv;
@@ -179,10 +244,11 @@ __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int lan
 // Restart helper: r = b - A x in FP64 for the wave's SpMV share, p = r (and
 // r^ = r for BiCGSTAB).  Returns the wave's share of ||r||^2.
 // lowv: the mixed CG's FP32 vectors r32 = p32 = (float) r instead.
+template <bool RES>
 __device__ __forceinline__ double restart_rows(const SolveArgs &a, int lane, bool mixed, bool bicg,
-                                               WaveLds *wl, bool lowv = false) {
+                                               WaveCtx<RES> &ctx, bool lowv = false) {
   double acc = 0.0;
-  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, ctx, lane, nullptr,
     [&](const int i, const double ax) {
       double rv = a.b[i] - ax;
       if (lowv) {
@@ -209,24 +275,28 @@ __device__ __forceinline__ double restart_rows(const SolveArgs &a, int lane, boo
 // FP64; FP64 tiles multiply the FP32 r in FP64 arithmetic); after a
 // fallback to FP64 tiles the recurrence continues in FP64 vectors.
 // Vectors: FP64 r, q (w), p, s / FP32 r32, q32 (w), p32, s32.
-template <int MODE>
+template <int MODE, bool RES>
 __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
-  __shared__ double lds[kWavesPerBlock];
+  __shared__ double lds[kMaxGridSum * kWavesPerBlock];
   __shared__ int lds_i;
-  __shared__ WaveLds wl[kWavesPerBlock];
+  __shared__ typename std::conditional<RES, ResLds, WaveLds>::type rl[kWavesPerBlock];
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  const int tr0 = a.wave_tr[wid], tr1 = a.wave_tr[wid + 1];
+  const int tr0 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid]);
+  const int tr1 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid + 1]);
   const int row0 = tr0 * 16, row1 = tr1 * 16;
   constexpr bool mixed = MODE != kAll64;
   const bool learned = MODE == kPerTile && a.mlp != nullptr;
   unsigned epoch = 0;
   int slot = 0;
   PrecCount cnt;
+  WaveCtx<RES> ctx;
+  ctx.L = &rl[threadIdx.x >> 6];
+  solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
 
   // r = b - A x0 (FP64, and the FP32 copy the mixed recurrence starts from)
   double red[2] = {0.0, 0.0};
-  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, ctx, lane, nullptr,
     [&](const int i, const double ax) {
       const double bv = a.b[i];
       double rv = bv - ax;
@@ -258,16 +328,25 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   bool done = bb == 0.0 || (a.fixed_iters <= 0 && rr <= stop2);
   bool pending = false;  // recompute the FP64 residual before the next step
   bool infer = learned;  // learned policy: new levels in the next SpMV
+  bool restage = false;  // rule policy: new segment levels in the next SpMV
   bool lowv = mixed;     // FP32 vectors (until a fallback to FP64 tiles)
 
+  int trace_it = -1;
+  (void)trace_it;
   while (!done) {
     double gamma = 0.0, delta = 0.0;
     if (!pending) {
-      // w = A r with gamma = r.r and delta = w.r of the wave's rows
+      ++trace_it;
+      MF_MARK(0);
+      // w = A r with gamma = r.r and delta = w.r of the wave's rows; new
+      // levels (inference, or a rule update before the last sync) are staged
+      // into a resident window first
       if (infer && !force64) wave_infer(a.A, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
+      if ((infer && !force64) || restage) solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
       infer = false;
+      restage = false;
       double gd[2] = {0.0, 0.0};
-      solver_spmv<MODE, mixed>(a, a.r, a.r32, a.seg_p, lowv ? 2 : force64, wl[threadIdx.x >> 6],
+      solver_spmv<MODE, mixed>(a, a.r, a.r32, a.seg_p, lowv ? 2 : force64, ctx,
                              lane, &cnt, [&](const int i, const double wv) {
         double rv;
         if (lowv) {
@@ -280,7 +359,9 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
         gd[0] += rv * rv;
         gd[1] += wv * rv;
       });
-      grid_sum<2>(gd, a.partials, a.gs, epoch, slot, lds, &lds_i);
+      MF_MARK(1);
+      grid_sum_own<RES>(a, gd, epoch, slot, lds, &lds_i);
+      MF_MARK(2);
       gamma = gd[0];
       delta = gd[1];
       rr = gamma;
@@ -297,8 +378,8 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       // FP64 residual replacement r = b - A x (x is complete: the last
       // update ended with a grid sync)
       const bool lv = lowv && !force64;
-      double tr[1] = {lv ? true_residual_rows(a, lane, nullptr, wl, a.r32)
-                         : true_residual_rows(a, lane, a.r, wl)};
+      double tr[1] = {lv ? true_residual_rows(a, lane, nullptr, ctx, a.r32)
+                         : true_residual_rows(a, lane, a.r, ctx)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
       it_ref = it;
@@ -320,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
         // the FP64 recurrence starts from an FP64 residual
         lowv = false;
         if (lv) {
-          double t2[1] = {true_residual_rows(a, lane, a.r, wl)};
+          double t2[1] = {true_residual_rows(a, lane, a.r, ctx)};
           grid_sum<1>(t2, a.partials, a.gs, epoch, slot, lds, &lds_i);
           rr = t2[0];
         }
@@ -331,10 +412,10 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
         if (lowv) {
           for (int i = row0 + lane; i < row1; i += 64) a.p[i] = (double)a.p32[i];
           grid_sync(a.gs, epoch, &lds_i);
-          solver_spmv<kAll64, false>(a, a.p, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane,
+          solver_spmv<kAll64, false>(a, a.p, nullptr, nullptr, 0, ctx, lane,
                                    nullptr, [&](const int i, const double sv) { a.s32[i] = (float)sv; });
         } else {
-          solver_spmv<kAll64, false>(a, a.p, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane,
+          solver_spmv<kAll64, false>(a, a.p, nullptr, nullptr, 0, ctx, lane,
                                    nullptr, [&](const int i, const double sv) { a.s[i] = sv; });
         }
       }
@@ -447,12 +528,15 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       }
     }
     infer = lupd;
+    restage = upd;
+    MF_MARK(3);
     grid_sync(a.gs, epoch, &lds_i);
+    MF_MARK(4);
     if (a.fixed_iters > 0 && it >= a.fixed_iters) break;
   }
-  if (mixed) flush_hist(cnt, a.hist, lds);
+  if (mixed) flush_hist(cnt, a.hist, lds, RES);
   // Final FP64 residual for the report.
-  double fin[1] = {true_residual_rows(a, lane, nullptr, wl)};
+  double fin[1] = {true_residual_rows(a, lane, nullptr, ctx)};
   grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.stats[kStIters] = it;
@@ -473,14 +557,15 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
 // breakdown.
 constexpr int kMaxRestart = 8;
 
-template <int MODE>
+template <int MODE, bool RES>
 __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
-  __shared__ double lds[kWavesPerBlock];
+  __shared__ double lds[kMaxGridSum * kWavesPerBlock];
   __shared__ int lds_i;
-  __shared__ WaveLds wl[kWavesPerBlock];
+  __shared__ typename std::conditional<RES, ResLds, WaveLds>::type rl[kWavesPerBlock];
   const int lane = threadIdx.x & 63;
   const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  const int tr0 = a.wave_tr[wid], tr1 = a.wave_tr[wid + 1];
+  const int tr0 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid]);
+  const int tr1 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid + 1]);
   const int row0 = tr0 * 16, row1 = tr1 * 16;
   constexpr bool mixed = MODE != kAll64;
   const bool learned = MODE == kPerTile && a.mlp != nullptr;
@@ -488,9 +573,14 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   int slot = 0;
   PrecCount cnt;
   double *v = a.q, *t = a.tv;
+  // (The rule policy multiplies p and s at different segment levels in one
+  // iteration: the host runs it with streamed windows.)
+  WaveCtx<RES> ctx;
+  ctx.L = &rl[threadIdx.x >> 6];
+  solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
 
   double red[2] = {0.0, 0.0};
-  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+  solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, ctx, lane, nullptr,
     [&](const int i, const double ax) {
       const double bv = a.b[i], rv = bv - ax;
       a.r[i] = rv;
@@ -523,7 +613,9 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   while (!done) {
     if (restart) {
       if (++restarts > kMaxRestart) { status = MF_ERR_BREAKDOWN; break; }
-      double tr[1] = {restart_rows(a, lane, mixed, true, wl)};
+      // x of every wave complete (the sums before this point order values only)
+      if constexpr (RES) grid_sync(a.gs, epoch, &lds_i);
+      double tr[1] = {restart_rows(a, lane, mixed, true, ctx)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
       rr = rho = tr[0];
@@ -535,14 +627,17 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     ++it;
     // v = A p, v.rh
     double vr[1] = {0.0};
-    if (infer && !force64) wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
+    if (infer && !force64) {
+      wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
+      solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
+    }
     infer = false;
-    solver_spmv<MODE, mixed>(a, a.p, a.p32, a.seg_p, force64, wl[threadIdx.x >> 6], lane, &cnt,
+    solver_spmv<MODE, mixed>(a, a.p, a.p32, a.seg_p, force64, ctx, lane, &cnt,
       [&](const int i, const double vv) {
         v[i] = vv;
         vr[0] += vv * a.rh[i];
       });
-    grid_sum<1>(vr, a.partials, a.gs, epoch, slot, lds, &lds_i);
+    grid_sum_own<RES>(a, vr, epoch, slot, lds, &lds_i);  // next: own rows of s
     if (!(vr[0] != 0.0) || !isfinite(vr[0])) {
       if (mixed) force64 = 1;
       restart = true;
@@ -580,13 +675,13 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     grid_sync(a.gs, epoch, &lds_i);
     // t = A s, t.s and t.t
     double tw[2] = {0.0, 0.0};
-    solver_spmv<MODE, mixed>(a, a.s, a.s32, a.seg_s, force64, wl[threadIdx.x >> 6], lane, &cnt,
+    solver_spmv<MODE, mixed>(a, a.s, a.s32, a.seg_s, force64, ctx, lane, &cnt,
       [&](const int i, const double tvv) {
         t[i] = tvv;
         tw[0] += tvv * a.s[i];
         tw[1] += tvv * tvv;
       });
-    grid_sum<2>(tw, a.partials, a.gs, epoch, slot, lds, &lds_i);
+    grid_sum_own<RES>(a, tw, epoch, slot, lds, &lds_i);  // next: own rows of x and r
     const double omega = uniform_double(tw[1] != 0.0 && isfinite(tw[1]) ? tw[0] / tw[1] : 0.0);
     // x += alpha p + omega s, r = s - omega t, rho' = r.rh, ||r||^2
     double nw[2] = {0.0, 0.0};
@@ -615,7 +710,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
         }
       }
     }
-    grid_sum<2>(nw, a.partials, a.gs, epoch, slot, lds, &lds_i);
+    grid_sum_own<RES>(a, nw, epoch, slot, lds, &lds_i);  // next: own rows of p (or a restart)
     double rho_new = nw[0], rr_new = nw[1];
     if (a.fixed_iters > 0) {
       if (it >= a.fixed_iters) { rr = rr_new; break; }
@@ -625,7 +720,8 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
                             (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
       if (converged || periodic || omega == 0.0 || !isfinite(rr_new)) {
         double tr[2] = {0.0, 0.0};
-        solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, wl[threadIdx.x >> 6], lane, nullptr,
+        if constexpr (RES) grid_sync(a.gs, epoch, &lds_i);  // x of every wave complete
+        solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, ctx, lane, nullptr,
           [&](const int i, const double ax) {
             const double rv = a.b[i] - ax;
             a.r[i] = rv;
@@ -696,8 +792,9 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
       grid_sync(a.gs, epoch, &lds_i);
     }
   }
-  if (mixed) flush_hist(cnt, a.hist, lds);
-  double fin[1] = {true_residual_rows(a, lane, nullptr, wl)};
+  if (mixed) flush_hist(cnt, a.hist, lds, RES);
+  if constexpr (RES) grid_sync(a.gs, epoch, &lds_i);  // x of every wave complete
+  double fin[1] = {true_residual_rows(a, lane, nullptr, ctx)};
   grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.stats[kStIters] = it;
@@ -747,7 +844,7 @@ __device__ __forceinline__ int opaque(int v) {
 template <int MODE, int K, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_cg_res(SolveArgs a) {
   constexpr int NW = BLOCK / 64;
-  __shared__ double lds[NW];
+  __shared__ double lds[kMaxGridSum * NW];
   __shared__ int lds_i;
   __shared__ double xbuf[NW * K * 64];
   const int lane = threadIdx.x & 63;
@@ -1016,6 +1113,7 @@ struct Workspace {
   uint8_t *seg_p, *seg_s;
   float *seg2;
   double *partials;
+  unsigned long long *mbox;
   GridSync *gs;
   double *stats;
   unsigned long long *hist;
@@ -1027,7 +1125,8 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
   size_t need = 6 * align256(n * 8) + 4 * align256(n * 4) + 2 * align256(segs) + align256(segs * 4) +
-                align256(2 * kMaxGridSum * (size_t)G * 8) + align256(sizeof(GridSync)) + align256(kStCount * 8) +
+                align256(2 * kMaxGridSum * (size_t)G * 8) + align256(((size_t)G + 1) * kMboxWords * 8) +
+                align256(sizeof(GridSync)) + align256(kStCount * 8) +
                 align256(5 * 8);
   if (t->work_bytes < need) {
     if (t->work) MF_HIP(hipFree(t->work));
@@ -1052,6 +1151,7 @@ mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   w.seg_s = (uint8_t *)take(segs);
   w.seg2 = (float *)take(segs * 4);
   w.partials = (double *)take(2 * kMaxGridSum * (size_t)G * 8);
+  w.mbox = (unsigned long long *)take(((size_t)G + 1) * kMboxWords * 8);
   w.gs = (GridSync *)take(sizeof(GridSync));
   w.stats = (double *)take(kStCount * 8);
   w.hist = (unsigned long long *)take(5 * 8);
@@ -1082,7 +1182,7 @@ mf_status grid_for(K kernel, int override_blocks, int *G, int block = kBlock, bo
 mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
   const int key = 2 * nwaves + (even_rows ? 1 : 0);
   if (t->sched.wave_tr && t->sched_waves_for == key) return MF_OK;
-  for (void *p : {(void *)t->sched.wave_tr, (void *)t->sched.wave_win, (void *)t->sched.win_desc,
+  for (void *p : {(void *)t->sched.wave_tr, (void *)t->sched.wave_res, (void *)t->sched.wave_win, (void *)t->sched.win_desc,
                   (void *)t->sched.heavy_part, (void *)t->sched.heavy_cnt, (void *)t->sched.heavy_n,
                   (void *)t->sched.heavy_base})
     if (p) MF_HIP(hipFree(p));
@@ -1160,10 +1260,28 @@ mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
     wd.insert(wd.end(), per[w].begin(), per[w].end());
     ww[w + 1] = (int)wd.size();
   }
+  // resident windows: a non-empty tile-row range without heavy tile-rows or
+  // pieces of other waves' heavy tile-rows, within the LDS window limits
+  std::vector<int> res(nwaves, 0);
+  for (int w = 0; w < nwaves; ++w) {
+    const int r0 = b[w], r1 = b[w + 1];
+    if (r1 <= r0) continue;
+    bool ok = true;
+    for (int r = r0; r < r1 && ok; ++r) ok = !heavy[r];
+    for (const WaveDesc &d : per[w]) ok = ok && !(d.flags & kDescPiece);
+    const int tiles = t->tile_ptr[r1] - t->tile_ptr[r0];
+    const int chunks = (t->pay_off[t->tile_ptr[r1]] - t->pay_off[t->tile_ptr[r0]]) / 4;
+    res[w] = ok && res_fits(r1 - r0, tiles, chunks) ? 1 : 0;
+  }
+  t->sched.all_resident = 1;
+  for (int w = 0; w < nwaves; ++w)
+    if (!res[w] && b[w + 1] > b[w]) t->sched.all_resident = 0;
   if (wd.empty()) wd.push_back(WaveDesc{0, 0, 0, 0, 0, 0, 0, 0});
   const int nh = (int)hn.size();
   MF_HIP(hipMalloc(&t->sched.wave_tr, sizeof(int) * (nwaves + 1)));
   MF_HIP(hipMemcpy(t->sched.wave_tr, b.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice));
+  MF_HIP(hipMalloc(&t->sched.wave_res, sizeof(int) * nwaves));
+  MF_HIP(hipMemcpy(t->sched.wave_res, res.data(), sizeof(int) * nwaves, hipMemcpyHostToDevice));
   MF_HIP(hipMalloc(&t->sched.wave_win, sizeof(int) * (nwaves + 1)));
   MF_HIP(hipMemcpy(t->sched.wave_win, ww.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice));
   MF_HIP(hipMalloc(&t->sched.win_desc, sizeof(WaveDesc) * wd.size()));
@@ -1216,7 +1334,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   const void *kern;
   int G = 0, block = kBlock;
   mf_status sst = MF_OK;
-  bool resident = false;
+  bool resident = false, resident_windows = false;
   if (opt.form < MF_FORM_AUTO || opt.form > MF_FORM_RESIDENT) return fail(MF_ERR_ARG, "solve: unknown form");
   // Auto picks the memory form: on MI355X (3D 7-point 128^3) it runs FP64 CG
   // as fast as the resident form (106 vs 108 us per iteration) and mixed CG
@@ -1256,16 +1374,35 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
     }
   }
   if (!resident) {
-    if (bicg)
-      kern = mode == kAll64 ? (const void *)k_bicgstab<kAll64>
-           : mode == kPerSeg ? (const void *)k_bicgstab<kPerSeg> : (const void *)k_bicgstab<kPerTile>;
-    else
-      kern = mode == kAll64 ? (const void *)k_cg<kAll64>
-           : mode == kPerSeg ? (const void *)k_cg<kPerSeg> : (const void *)k_cg<kPerTile>;
-    sst = grid_for(kern, opt.blocks, &G);
+    // The grid is the occupancy of the streamed kernel (one block of 16
+    // waves per CU for both forms); the resident kernel runs when every
+    // wave's share fits a resident window (not for the rule policy's
+    // BiCGSTAB, which multiplies p and s at different segment levels).
+    const void *kstream = bicg ? (mode == kAll64 ? (const void *)k_bicgstab<kAll64, false>
+                                  : mode == kPerSeg ? (const void *)k_bicgstab<kPerSeg, false>
+                                                    : (const void *)k_bicgstab<kPerTile, false>)
+                               : (mode == kAll64 ? (const void *)k_cg<kAll64, false>
+                                  : mode == kPerSeg ? (const void *)k_cg<kPerSeg, false>
+                                                    : (const void *)k_cg<kPerTile, false>);
+    sst = grid_for(kstream, opt.blocks, &G);
     if (sst != MF_OK) return sst;
     sst = ensure_schedule(t, G * kWavesPerBlock);
     if (sst != MF_OK) return sst;
+    kern = kstream;
+    if (t->sched.all_resident && !(bicg && mode == kPerSeg) && !std::getenv("MF_NO_RESIDENT_WINDOWS")) {
+      int Gr = 0;
+      const void *kres = bicg ? (mode == kAll64 ? (const void *)k_bicgstab<kAll64, true>
+                                                : (const void *)k_bicgstab<kPerTile, true>)
+                              : (mode == kAll64 ? (const void *)k_cg<kAll64, true>
+                                 : mode == kPerSeg ? (const void *)k_cg<kPerSeg, true>
+                                                   : (const void *)k_cg<kPerTile, true>);
+      sst = grid_for(kres, opt.blocks, &Gr);
+      if (sst != MF_OK) return sst;
+      if (Gr == G) {
+        kern = kres;
+        resident_windows = true;
+      }
+    }
   }
   Workspace w;
   sst = get_workspace(t, G, w);
@@ -1273,6 +1410,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
 
   const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
   MF_HIP(hipMemsetAsync(w.gs, 0, sizeof(GridSync), st));
+  MF_HIP(hipMemsetAsync(w.mbox, 0, ((size_t)G + 1) * kMboxWords * 8, st));
   if (t->sched.nheavy > 0)
     MF_HIP(hipMemsetAsync(t->sched.heavy_cnt, 0, sizeof(unsigned) * t->sched.nheavy, st));
   MF_HIP(hipMemsetAsync(w.hist, 0, 5 * 8, st));
@@ -1306,6 +1444,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   a.heavy_n = t->sched.heavy_n;
   a.heavy_base = t->sched.heavy_base;
   a.partials = w.partials;
+  a.mbox = w.mbox;
   a.gs = w.gs;
   a.stats = w.stats;
   a.hist = w.hist;
@@ -1325,6 +1464,15 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   a.nseg = (int)segs;
   if (learned) MF_HIP(hipMemsetAsync(w.seg2, 0, segs * 4, st));
 
+#ifdef MF_PHASE_TRACE
+  const char *trace_file = std::getenv("MF_PHASE_TRACE_FILE");
+  const size_t trace_n = (size_t)kTraceIters * kTracePhases * G;
+  a.trace = nullptr;
+  if (trace_file && !bicg) {
+    MF_HIP(hipMalloc(&a.trace, trace_n * 8));
+    MF_HIP(hipMemsetAsync(a.trace, 0, trace_n * 8, st));
+  }
+#endif
   hipEvent_t e0, e1;
   MF_HIP(hipEventCreate(&e0));
   MF_HIP(hipEventCreate(&e1));
@@ -1352,6 +1500,19 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
 
+#ifdef MF_PHASE_TRACE
+  if (a.trace) {
+    std::vector<unsigned long long> tr(trace_n);
+    MF_HIP(hipMemcpy(tr.data(), a.trace, trace_n * 8, hipMemcpyDeviceToHost));
+    MF_HIP(hipFree(a.trace));
+    if (FILE *f = std::fopen(trace_file, "ab")) {
+      const long long hdr[4] = {G, kTraceIters, kTracePhases, opt.policy};
+      std::fwrite(hdr, 8, 4, f);
+      std::fwrite(tr.data(), 8, trace_n, f);
+      std::fclose(f);
+    }
+  }
+#endif
   double stats[kStCount];
   unsigned long long hist[5];
   GridSync gsh;

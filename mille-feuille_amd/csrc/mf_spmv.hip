@@ -21,7 +21,7 @@ namespace mf {
 // PIECES: the schedule has heavy tile-rows cut into pieces (wave_spmv_piece);
 // without them the kernel keeps the piece bookkeeping out of its registers.
 template <int MODE, bool PIECES>
-__global__ __launch_bounds__(256, MODE == kAll64 ? 8 : 6) void k_spmv(TileView A, const WaveDesc *__restrict__ desc,
+__global__ __launch_bounds__(256, MODE == kAll64 ? 8 : 5) void k_spmv(TileView A, const WaveDesc *__restrict__ desc,
                                               int nwaves, HeavyView hv, const double *__restrict__ x,
                                               const float *__restrict__ x32,
                                               double *__restrict__ y) {
