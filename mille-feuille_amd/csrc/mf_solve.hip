@@ -49,6 +49,17 @@ enum StatSlot {
   kStCount = 16
 };
 
+// Wave index of the calling wave in the solver's tile-row order.  Blocks are
+// dispatched round-robin over the 8 XCDs (block b on XCD b % 8); with a grid
+// that is a multiple of 8, XCD x takes the x-th eighth of the waves, so the
+// tile-rows a wave's neighbours own (and whose vector entries its SpMV
+// gathers) sit in the same XCD's L2.
+__device__ __forceinline__ int solver_wave_id() {
+  const int G = gridDim.x;
+  const int lb = (G & 7) ? (int)blockIdx.x : (int)(blockIdx.x & 7) * (G >> 3) + (int)(blockIdx.x >> 3);
+  return lb * kWavesPerBlock + (int)(threadIdx.x >> 6);
+}
+
 struct SolveArgs {
   TileView A;
   const double *b;
@@ -189,7 +200,7 @@ __device__ __forceinline__ void solver_spmv(const SolveArgs &a, const double *__
     return;
   }
   WaveLds &L = *reinterpret_cast<WaveLds *>(ctx.L);
-  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int wid = solver_wave_id();
   const int d0 = __builtin_amdgcn_readfirstlane(a.wave_win[wid]);
   const int d1 = __builtin_amdgcn_readfirstlane(a.wave_win[wid + 1]);
   const HeavyView hv{a.heavy_part, a.heavy_cnt, a.heavy_n, a.heavy_base};
@@ -281,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   __shared__ int lds_i;
   __shared__ typename std::conditional<RES, ResLds, WaveLds>::type rl[kWavesPerBlock];
   const int lane = threadIdx.x & 63;
-  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int wid = solver_wave_id();
   const int tr0 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid]);
   const int tr1 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid + 1]);
   const int row0 = tr0 * 16, row1 = tr1 * 16;
@@ -563,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   __shared__ int lds_i;
   __shared__ typename std::conditional<RES, ResLds, WaveLds>::type rl[kWavesPerBlock];
   const int lane = threadIdx.x & 63;
-  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int wid = solver_wave_id();
   const int tr0 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid]);
   const int tr1 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid + 1]);
   const int row0 = tr0 * 16, row1 = tr1 * 16;
@@ -1191,7 +1202,9 @@ mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
   const int tm = t->tilem;
   auto slots = [&](int r) { return t->pay_off[t->tile_ptr[r + 1]] - t->pay_off[t->tile_ptr[r]]; };
   auto tiles = [&](int r) { return t->tile_ptr[r + 1] - t->tile_ptr[r]; };
-  auto cost = [&](int r) { return (double)slots(r) + 8.0 * tiles(r) + 16.0; };
+  // a tile-row's work: its payload slots and tiles in the SpMV, and its 16
+  // rows in the vector updates (about 128 slots' worth)
+  auto cost = [&](int r) { return (double)slots(r) + 8.0 * tiles(r) + 128.0; };
   std::vector<char> heavy(tm, 0);
   std::vector<int> b;
   if (even_rows) {
