@@ -267,6 +267,19 @@ void mf_solve_opts_default(mf_solve_opts *o);
 mf_status mf_cg_solve(mf_tiled *t, const double *d_b, double *d_x,
                       const mf_solve_opts *opts, mf_report *rep);
 
+/* mf_cg_solve without any device allocation.  No counterpart in the
+ * reference: this is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * work vectors, grid-barrier state and the solver schedule live in one
+ * library-owned device array of 12 MiB per device (held until the process
+ * ends), which limits the system to about 150,000 rows; larger systems are
+ * rejected with MF_ERR_ARG.  The caller passes d_b and d_x as for
+ * mf_cg_solve.  Every call on a device shares that array, so calls that
+ * overlap on one device race; the report's time_ms is 0 (no events are
+ * created); the resident register form (MF_FORM_RESIDENT) is not used. */
+mf_status mf_cg_solve_noalloc(mf_tiled *t, const double *d_b, double *d_x,
+                              const mf_solve_opts *opts, mf_report *rep);
+
 /* BiCGSTAB in one persistent cooperative kernel, same conventions.
  * Replaces bicgstab_sync / bicgstab_inc (src/main-bicg.cu:13, 711), the
  * mixed bicgstab (src/main-bicg-mixed.cu:1355) and their kernels

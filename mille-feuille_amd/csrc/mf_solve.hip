@@ -20,6 +20,8 @@
 // i.e. iterative refinement inside one Krylov run).
 #include <algorithm>
 #include <queue>
+#include <mutex>
+#include <string>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1190,15 +1192,15 @@ mf_status grid_for(K kernel, int override_blocks, int *G, int block = kBlock, bo
 // cut into pieces spread over the waves with the least work, so hub rows of
 // power-law matrices do not serialise on one wave.  Cached per (wave count,
 // kind).
-mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
-  const int key = 2 * nwaves + (even_rows ? 1 : 0);
-  if (t->sched.wave_tr && t->sched_waves_for == key) return MF_OK;
-  for (void *p : {(void *)t->sched.wave_tr, (void *)t->sched.wave_res, (void *)t->sched.wave_win, (void *)t->sched.win_desc,
-                  (void *)t->sched.heavy_part, (void *)t->sched.heavy_cnt, (void *)t->sched.heavy_n,
-                  (void *)t->sched.heavy_base})
-    if (p) MF_HIP(hipFree(p));
-  t->sched = Schedule();
-  t->sched_waves_for = -1;
+// Host side of a solver schedule: per-wave tile-row bounds, resident flags,
+// the waves' SpMV descriptors and the heavy tile-row pieces.
+struct HostSchedule {
+  std::vector<int> b, res, ww, hn, hbase;
+  std::vector<WaveDesc> wd;
+  int npieces = 0, all_resident = 1;
+};
+
+void build_schedule(const mf_tiled *t, int nwaves, bool even_rows, HostSchedule &hsch) {
   const int tm = t->tilem;
   auto slots = [&](int r) { return t->pay_off[t->tile_ptr[r + 1]] - t->pay_off[t->tile_ptr[r]]; };
   auto tiles = [&](int r) { return t->tile_ptr[r + 1] - t->tile_ptr[r]; };
@@ -1286,10 +1288,34 @@ mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
     const int chunks = (t->pay_off[t->tile_ptr[r1]] - t->pay_off[t->tile_ptr[r0]]) / 4;
     res[w] = ok && res_fits(r1 - r0, tiles, chunks) ? 1 : 0;
   }
-  t->sched.all_resident = 1;
+  hsch.all_resident = 1;
   for (int w = 0; w < nwaves; ++w)
-    if (!res[w] && b[w + 1] > b[w]) t->sched.all_resident = 0;
+    if (!res[w] && b[w + 1] > b[w]) hsch.all_resident = 0;
   if (wd.empty()) wd.push_back(WaveDesc{0, 0, 0, 0, 0, 0, 0, 0});
+  hsch.b = std::move(b);
+  hsch.res = std::move(res);
+  hsch.ww = std::move(ww);
+  hsch.wd = std::move(wd);
+  hsch.hn = std::move(hn);
+  hsch.hbase = std::move(hbase);
+  hsch.npieces = npieces;
+}
+
+mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
+  const int key = 2 * nwaves + (even_rows ? 1 : 0);
+  if (t->sched.wave_tr && t->sched_waves_for == key) return MF_OK;
+  for (void *p : {(void *)t->sched.wave_tr, (void *)t->sched.wave_res, (void *)t->sched.wave_win, (void *)t->sched.win_desc,
+                  (void *)t->sched.heavy_part, (void *)t->sched.heavy_cnt, (void *)t->sched.heavy_n,
+                  (void *)t->sched.heavy_base})
+    if (p) MF_HIP(hipFree(p));
+  t->sched = Schedule();
+  t->sched_waves_for = -1;
+  HostSchedule hsch;
+  build_schedule(t, nwaves, even_rows, hsch);
+  const std::vector<int> &b = hsch.b, &res = hsch.res, &ww = hsch.ww, &hn = hsch.hn, &hbase = hsch.hbase;
+  const std::vector<WaveDesc> &wd = hsch.wd;
+  const int npieces = hsch.npieces;
+  t->sched.all_resident = hsch.all_resident;
   const int nh = (int)hn.size();
   MF_HIP(hipMalloc(&t->sched.wave_tr, sizeof(int) * (nwaves + 1)));
   MF_HIP(hipMemcpy(t->sched.wave_tr, b.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice));
@@ -1323,8 +1349,120 @@ int spmv_mode_for(int policy) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Allocation-free solves (mf_cg_solve_noalloc): the workspace and schedule
+// live in one file-scope device array of kScratchBytes per device, whose
+// address (hipGetSymbolAddress, once per device) replaces the handle's
+// allocations.  Calls on one device share it: they must not overlap.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)12 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kScratchDevices] = {};
+
+// The scratch array's device address on device `dev` (current).
+mf_status scratch_address(int dev, unsigned char **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0 || dev >= kScratchDevices) return fail(MF_ERR_ARG, "noalloc: device index out of range");
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(MF_ERR_HIP, std::string("noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = (unsigned char *)g_scratch_ptr[dev];
+  return MF_OK;
+}
+
+// Fill of n 32-bit words (a kernel, not a memset: stays valid under capture).
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+mf_status fill_u32(void *p, size_t bytes, uint32_t v, hipStream_t st) {
+  const size_t n = bytes / 4;
+  if (n == 0) return MF_OK;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
+  k_fill_u32<<<blocks, 256, 0, st>>>((uint32_t *)p, n, v);
+  MF_HIP(hipGetLastError());
+  return MF_OK;
+}
+
+// Carves the workspace and the schedule of a solve out of the scratch array;
+// fails when they do not fit.  Host schedule arrays are copied in.
+mf_status scratch_layout(const mf_tiled *t, int G, const HostSchedule &hs, unsigned char *base,
+                         Workspace &w, Schedule &sc, hipStream_t st) {
+  const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
+  const int nwaves = (int)hs.b.size() - 1, nh = (int)hs.hn.size();
+  size_t off = 0;
+  auto take = [&](size_t bytes) { const size_t o = off; off += align256(bytes); return o; };
+  const size_t o_vec = take(6 * align256(n * 8) + 4 * align256(n * 4));
+  const size_t o_segp = take(segs), o_segs = take(segs), o_seg2 = take(segs * 4);
+  const size_t o_part = take(2 * kMaxGridSum * (size_t)G * 8);
+  const size_t o_mbox = take(((size_t)G + 1) * kMboxWords * 8);
+  const size_t o_gs = take(sizeof(GridSync)), o_stats = take(kStCount * 8), o_hist = take(5 * 8);
+  const size_t o_tr = take(sizeof(int) * (nwaves + 1)), o_res = take(sizeof(int) * nwaves);
+  const size_t o_win = take(sizeof(int) * (nwaves + 1)), o_desc = take(sizeof(WaveDesc) * hs.wd.size());
+  const size_t o_hpart = take(sizeof(double) * 16 * std::max(hs.npieces, 1));
+  const size_t o_hcnt = take(sizeof(unsigned) * std::max(nh, 1)), o_hn = take(sizeof(int) * std::max(nh, 1));
+  const size_t o_hbase = take(sizeof(int) * std::max(nh, 1));
+  if (off > kScratchBytes)
+    return fail(MF_ERR_ARG, "mf_cg_solve_noalloc: the system needs " + std::to_string(off) +
+                                " bytes of workspace, more than the " + std::to_string(kScratchBytes) +
+                                " bytes of the allocation-free scratch (about 150,000 rows)");
+  unsigned char *v = base + o_vec;
+  w.r = (double *)v; v += align256(n * 8);
+  w.p = (double *)v; v += align256(n * 8);
+  w.q = (double *)v; v += align256(n * 8);
+  w.rh = (double *)v; v += align256(n * 8);
+  w.s = (double *)v; v += align256(n * 8);
+  w.tv = (double *)v; v += align256(n * 8);
+  w.p32 = (float *)v; v += align256(n * 4);
+  w.s32 = (float *)v; v += align256(n * 4);
+  w.r32 = (float *)v; v += align256(n * 4);
+  w.q32 = (float *)v;
+  w.seg_p = base + o_segp;
+  w.seg_s = base + o_segs;
+  w.seg2 = (float *)(base + o_seg2);
+  w.partials = (double *)(base + o_part);
+  w.mbox = (unsigned long long *)(base + o_mbox);
+  w.gs = (GridSync *)(base + o_gs);
+  w.stats = (double *)(base + o_stats);
+  w.hist = (unsigned long long *)(base + o_hist);
+  sc = Schedule();
+  sc.nwaves = nwaves;
+  sc.nheavy = nh;
+  sc.all_resident = hs.all_resident;
+  sc.wave_tr = (int *)(base + o_tr);
+  sc.wave_res = (int *)(base + o_res);
+  sc.wave_win = (int *)(base + o_win);
+  sc.win_desc = (WaveDesc *)(base + o_desc);
+  sc.heavy_part = (double *)(base + o_hpart);
+  sc.heavy_cnt = (unsigned *)(base + o_hcnt);
+  sc.heavy_n = (int *)(base + o_hn);
+  sc.heavy_base = (int *)(base + o_hbase);
+  MF_HIP(hipMemcpyAsync(sc.wave_tr, hs.b.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice, st));
+  MF_HIP(hipMemcpyAsync(sc.wave_res, hs.res.data(), sizeof(int) * nwaves, hipMemcpyHostToDevice, st));
+  MF_HIP(hipMemcpyAsync(sc.wave_win, hs.ww.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice, st));
+  MF_HIP(hipMemcpyAsync(sc.win_desc, hs.wd.data(), sizeof(WaveDesc) * hs.wd.size(), hipMemcpyHostToDevice, st));
+  if (nh > 0) {
+    MF_HIP(hipMemcpyAsync(sc.heavy_n, hs.hn.data(), sizeof(int) * nh, hipMemcpyHostToDevice, st));
+    MF_HIP(hipMemcpyAsync(sc.heavy_base, hs.hbase.data(), sizeof(int) * nh, hipMemcpyHostToDevice, st));
+  }
+  // the scratch keeps the last call's contents: clear what starts at zero
+  MF_TRY(fill_u32(base + o_vec, o_segp - o_vec, 0u, st));  // vectors (zero padding)
+  MF_TRY(fill_u32(base + o_segp, o_part - o_segp, 0u, st));  // segment levels (FP64 = 0), norms
+  MF_TRY(fill_u32(base + o_mbox, o_tr - o_mbox, 0u, st));  // mailboxes, barrier, stats, histogram
+  MF_TRY(fill_u32(base + o_hcnt, align256(sizeof(unsigned) * std::max(nh, 1)), 0u, st));
+  return MF_OK;
+}
+
 mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_solve_opts *o,
-                         mf_report *rep, bool bicg) {
+                         mf_report *rep, bool bicg, bool noalloc = false) {
   if (!t || !d_b || !d_x) return fail(MF_ERR_ARG, "solve: null argument");
   if (t->dev.device < 0) return fail(MF_ERR_ARG, "solve: host-only handle");
   mf_solve_opts opt;
@@ -1348,12 +1486,13 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   int G = 0, block = kBlock;
   mf_status sst = MF_OK;
   bool resident = false, resident_windows = false;
+  HostSchedule nsched;  // allocation-free solves: the schedule built on the host
   if (opt.form < MF_FORM_AUTO || opt.form > MF_FORM_RESIDENT) return fail(MF_ERR_ARG, "solve: unknown form");
   // Auto picks the memory form: on MI355X (3D 7-point 128^3) it runs FP64 CG
   // as fast as the resident form (106 vs 108 us per iteration) and mixed CG
   // faster (152 vs 213): the resident form's 8-16 waves per CU leave the SpMV
   // latency-bound.
-  if (!bicg && opt.form == MF_FORM_RESIDENT && regular_tilerows(t)) {
+  if (!bicg && !noalloc && opt.form == MF_FORM_RESIDENT && regular_tilerows(t)) {
     // Resident CG when every wave's even share of rows fits its registers:
     // 16 waves x 512 rows per CU unless that kernel spills, else 8 waves x
     // 1024 rows.
@@ -1399,10 +1538,15 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
                                                     : (const void *)k_cg<kPerTile, false>);
     sst = grid_for(kstream, opt.blocks, &G);
     if (sst != MF_OK) return sst;
-    sst = ensure_schedule(t, G * kWavesPerBlock);
-    if (sst != MF_OK) return sst;
+    if (noalloc) {
+      build_schedule(t, G * kWavesPerBlock, false, nsched);
+    } else {
+      sst = ensure_schedule(t, G * kWavesPerBlock);
+      if (sst != MF_OK) return sst;
+    }
     kern = kstream;
-    if (t->sched.all_resident && !(bicg && mode == kPerSeg) && !std::getenv("MF_NO_RESIDENT_WINDOWS")) {
+    const int all_res = noalloc ? nsched.all_resident : t->sched.all_resident;
+    if (all_res && !(bicg && mode == kPerSeg) && !std::getenv("MF_NO_RESIDENT_WINDOWS")) {
       int Gr = 0;
       const void *kres = bicg ? (mode == kAll64 ? (const void *)k_bicgstab<kAll64, true>
                                                 : (const void *)k_bicgstab<kPerTile, true>)
@@ -1418,22 +1562,30 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
     }
   }
   Workspace w;
-  sst = get_workspace(t, G, w);
-  if (sst != MF_OK) return sst;
-
+  Schedule scr_sched;
+  const Schedule *sch = &t->sched;
   const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
-  MF_HIP(hipMemsetAsync(w.gs, 0, sizeof(GridSync), st));
-  MF_HIP(hipMemsetAsync(w.mbox, 0, ((size_t)G + 1) * kMboxWords * 8, st));
-  if (t->sched.nheavy > 0)
-    MF_HIP(hipMemsetAsync(t->sched.heavy_cnt, 0, sizeof(unsigned) * t->sched.nheavy, st));
-  MF_HIP(hipMemsetAsync(w.hist, 0, 5 * 8, st));
-  MF_HIP(hipMemsetAsync(w.stats, 0, kStCount * 8, st));
-  MF_HIP(hipMemsetAsync(w.seg_p, MF_PREC_FP64, segs, st));
-  MF_HIP(hipMemsetAsync(w.seg_s, MF_PREC_FP64, segs, st));
-  // Padding entries of the work vectors must be zero: tiles whose columns
-  // lie beyond the rows gather them.
-  MF_HIP(hipMemsetAsync(w.r, 0, 6 * align256(n * 8), st));
-  MF_HIP(hipMemsetAsync(w.p32, 0, 4 * align256(n * 4), st));
+  if (noalloc) {
+    unsigned char *base = nullptr;
+    MF_TRY(scratch_address(t->dev.device, &base));
+    MF_TRY(scratch_layout(t, G, nsched, base, w, scr_sched, st));
+    sch = &scr_sched;
+  } else {
+    sst = get_workspace(t, G, w);
+    if (sst != MF_OK) return sst;
+    MF_HIP(hipMemsetAsync(w.gs, 0, sizeof(GridSync), st));
+    MF_HIP(hipMemsetAsync(w.mbox, 0, ((size_t)G + 1) * kMboxWords * 8, st));
+    if (t->sched.nheavy > 0)
+      MF_HIP(hipMemsetAsync(t->sched.heavy_cnt, 0, sizeof(unsigned) * t->sched.nheavy, st));
+    MF_HIP(hipMemsetAsync(w.hist, 0, 5 * 8, st));
+    MF_HIP(hipMemsetAsync(w.stats, 0, kStCount * 8, st));
+    MF_HIP(hipMemsetAsync(w.seg_p, MF_PREC_FP64, segs, st));
+    MF_HIP(hipMemsetAsync(w.seg_s, MF_PREC_FP64, segs, st));
+    // Padding entries of the work vectors must be zero: tiles whose columns
+    // lie beyond the rows gather them.
+    MF_HIP(hipMemsetAsync(w.r, 0, 6 * align256(n * 8), st));
+    MF_HIP(hipMemsetAsync(w.p32, 0, 4 * align256(n * 4), st));
+  }
   if (opt.policy == MF_POLICY_FIXED)
     MF_HIP(hipMemcpyAsync(t->dev.prec, t->dev.prec_user, t->tilenum, hipMemcpyDeviceToDevice, st));
   if (opt.policy == MF_POLICY_ERRBOUND && t->tilenum > 0) {
@@ -1449,13 +1601,13 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   a.r = w.r; a.p = w.p; a.q = w.q; a.rh = w.rh; a.s = w.s; a.tv = w.tv;
   a.p32 = w.p32; a.s32 = w.s32; a.r32 = w.r32; a.q32 = w.q32;
   a.seg_p = w.seg_p; a.seg_s = w.seg_s;
-  a.wave_tr = t->sched.wave_tr;
-  a.wave_win = t->sched.wave_win;
-  a.win_desc = t->sched.win_desc;
-  a.heavy_part = t->sched.heavy_part;
-  a.heavy_cnt = t->sched.heavy_cnt;
-  a.heavy_n = t->sched.heavy_n;
-  a.heavy_base = t->sched.heavy_base;
+  a.wave_tr = sch->wave_tr;
+  a.wave_win = sch->wave_win;
+  a.win_desc = sch->win_desc;
+  a.heavy_part = sch->heavy_part;
+  a.heavy_cnt = sch->heavy_cnt;
+  a.heavy_n = sch->heavy_n;
+  a.heavy_base = sch->heavy_base;
   a.partials = w.partials;
   a.mbox = w.mbox;
   a.gs = w.gs;
@@ -1475,7 +1627,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   a.seg2 = w.seg2;
   a.prec_rw = t->dev.prec;
   a.nseg = (int)segs;
-  if (learned) MF_HIP(hipMemsetAsync(w.seg2, 0, segs * 4, st));
+  if (learned && !noalloc) MF_HIP(hipMemsetAsync(w.seg2, 0, segs * 4, st));
 
 #ifdef MF_PHASE_TRACE
   const char *trace_file = std::getenv("MF_PHASE_TRACE_FILE");
@@ -1486,10 +1638,12 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
     MF_HIP(hipMemsetAsync(a.trace, 0, trace_n * 8, st));
   }
 #endif
-  hipEvent_t e0, e1;
-  MF_HIP(hipEventCreate(&e0));
-  MF_HIP(hipEventCreate(&e1));
-  MF_HIP(hipEventRecord(e0, st));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!noalloc) {
+    MF_HIP(hipEventCreate(&e0));
+    MF_HIP(hipEventCreate(&e1));
+    MF_HIP(hipEventRecord(e0, st));
+  }
   void *args[] = {&a};
   // The grid never exceeds what the occupancy query says is resident at once.
   // A cooperative launch also guarantees it; when the runtime refuses one
@@ -1502,16 +1656,20 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
     le = hipLaunchKernel(kern, dim3(G), dim3(block), args, 0, st);
   }
   if (le != hipSuccess) {
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     return fail(MF_ERR_COOP, std::string("hipLaunchCooperativeKernel: ") + hipGetErrorString(le));
   }
-  MF_HIP(hipEventRecord(e1, st));
-  MF_HIP(hipEventSynchronize(e1));
   float ms = 0.0f;
-  MF_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (noalloc) {
+    MF_HIP(hipStreamSynchronize(st));  // (no events: the report's time is 0)
+  } else {
+    MF_HIP(hipEventRecord(e1, st));
+    MF_HIP(hipEventSynchronize(e1));
+    MF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
 
 #ifdef MF_PHASE_TRACE
   if (a.trace) {
@@ -1580,4 +1738,9 @@ extern "C" mf_status mf_cg_solve(mf_tiled *t, const double *d_b, double *d_x,
 extern "C" mf_status mf_bicgstab_solve(mf_tiled *t, const double *d_b, double *d_x,
                                        const mf_solve_opts *opts, mf_report *rep) {
   return run_persistent(t, d_b, d_x, opts, rep, true);
+}
+
+extern "C" mf_status mf_cg_solve_noalloc(mf_tiled *t, const double *d_b, double *d_x,
+                                        const mf_solve_opts *opts, mf_report *rep) {
+  return run_persistent(t, d_b, d_x, opts, rep, false, true);
 }

@@ -117,6 +117,7 @@ SIGNATURES = {
     "mf_tile_set_spmv_kernel": (C.c_int, [_vp, C.c_int]),
     "mf_solve_opts_default": (None, [C.POINTER(MfSolveOpts)]),
     "mf_cg_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(MfSolveOpts), C.POINTER(MfReport)]),
+    "mf_cg_solve_noalloc": (C.c_int, [_vp, _vp, _vp, C.POINTER(MfSolveOpts), C.POINTER(MfReport)]),
     "mf_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(MfSolveOpts), C.POINTER(MfReport)]),
     "mf_cg_solve_multikernel": (C.c_int, [_vp, _vp, _vp, C.POINTER(MfSolveOpts), C.POINTER(MfReport)]),
     "mf_bicgstab_solve_multikernel": (C.c_int, [_vp, _vp, _vp, C.POINTER(MfSolveOpts), C.POINTER(MfReport)]),

@@ -241,6 +241,12 @@ class TiledMatrix:
         allow = (0, 1) if allow_not_converged else (0,)
         return self._solve(lib.mf_cg_solve, d_b, d_x, solve_opts(**kw), allow)
 
+    def cg_noalloc(self, d_b, d_x, allow_not_converged=False, **kw):
+        """mf_cg_solve_noalloc: the same solve with no device allocation (the
+        library's fixed scratch array; systems up to about 150,000 rows)."""
+        allow = (0, 1) if allow_not_converged else (0,)
+        return self._solve(lib.mf_cg_solve_noalloc, d_b, d_x, solve_opts(**kw), allow)
+
     def bicgstab(self, d_b, d_x, allow_not_converged=False, **kw):
         allow = (0, 1) if allow_not_converged else (0,)
         return self._solve(lib.mf_bicgstab_solve, d_b, d_x, solve_opts(**kw), allow)
