@@ -1387,14 +1387,13 @@ __device__ __forceinline__ void rowgroup_passes(const TileView &A, int tr0, int 
   }
 }
 
-// Learned tile levels for the wave's tile-rows [tr0, tr1): the MLP runs on 16
-// tiles per MFMA batch and the levels go to ic.prec_out, where the next
-// per-tile SpMV pass of the same wave reads them (the closing workgroup
-// fence orders the stores before those loads).
-__device__ __forceinline__ void wave_infer(const TileView &A, const InferCtx &ic, int tr0, int tr1,
-                                           int lane) {
-  if (tr0 >= tr1) return;
-  const int t0 = A.tile_ptr[tr0], t1 = A.tile_ptr[tr1];
+// Learned tile levels for tiles [t0, t1): the MLP runs on 16 tiles per MFMA
+// batch and the levels go to ic.prec_out, where the next per-tile SpMV pass
+// of the same wave reads them (the closing workgroup fence orders the stores
+// before those loads).  Callers infer exactly the tiles their own SpMV reads,
+// so no level is written while another wave may read it.
+__device__ __forceinline__ void wave_infer_tiles(const TileView &A, const InferCtx &ic, int t0, int t1,
+                                                 int lane) {
   for (int g = t0; g < t1; g += 16) {
     const int t = g + (lane & 15);
     const bool valid = t < t1;
@@ -1404,6 +1403,13 @@ __device__ __forceinline__ void wave_infer(const TileView &A, const InferCtx &ic
     if (lane < 16 && valid) ic.prec_out[t] = (uint8_t)mlp_argmax(z);
   }
   __threadfence_block();
+}
+
+// Learned tile levels for the tiles of tile-rows [tr0, tr1).
+__device__ __forceinline__ void wave_infer(const TileView &A, const InferCtx &ic, int tr0, int tr1,
+                                           int lane) {
+  if (tr0 >= tr1) return;
+  wave_infer_tiles(A, ic, A.tile_ptr[tr0], A.tile_ptr[tr1], lane);
 }
 
 // Deterministic sum over the 64 lanes of a wave (fixed xor tree); every lane

@@ -106,6 +106,8 @@ struct Schedule {
   unsigned *heavy_cnt = nullptr;  // device, nheavy
   int *heavy_n = nullptr;         // device, nheavy
   int *heavy_base = nullptr;      // device, nheavy
+  int *heavy_tr = nullptr;        // device, nheavy: tile-row of each heavy tile-row
+  int *wave_heavy = nullptr;      // device, nwaves+1: heavy tile-rows each wave owns
 };
 
 }  // namespace mf

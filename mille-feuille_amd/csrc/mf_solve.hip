@@ -78,6 +78,8 @@ struct SolveArgs {
   unsigned *heavy_cnt;       // pieces of each heavy tile-row done
   const int *heavy_n;        // pieces of each heavy tile-row
   const int *heavy_base;     // first piece slot of each heavy tile-row
+  const int *heavy_tr;       // tile-row of each heavy tile-row
+  const int *wave_heavy;     // per wave, bounds of the heavy tile-rows it owns
   double *partials;
   unsigned long long *mbox;   // value-only grid sums: (blocks + 1) mailboxes
   GridSync *gs;
@@ -205,10 +207,79 @@ __device__ __forceinline__ void solver_spmv(const SolveArgs &a, const double *__
   const int wid = solver_wave_id();
   const int d0 = __builtin_amdgcn_readfirstlane(a.wave_win[wid]);
   const int d1 = __builtin_amdgcn_readfirstlane(a.wave_win[wid + 1]);
-  const HeavyView hv{a.heavy_part, a.heavy_cnt, a.heavy_n, a.heavy_base};
-  for (int d = d0; d < d1; ++d)
-    wave_spmv_piece<MODE, COUNT, 3, false, MODE != kAll64>(a.A, a.win_desc[d], hv, x, x32, seglev,
-                                                          force64, L, lane, cnt, epi);
+  for (int d = d0; d < d1; ++d) {
+    const WaveDesc desc = a.win_desc[d];
+    const int flags = __builtin_amdgcn_readfirstlane(desc.flags);
+    if (!(flags & kDescPiece)) {
+      wave_spmv<MODE, COUNT, 3, false, MODE != kAll64>(a.A, desc, x, x32, seglev, force64, L, lane,
+                                                       cnt, epi);
+      continue;
+    }
+    // A piece of a heavy tile-row: its 16 partial row sums go to its slot
+    // (agent-scope stores, visible across XCDs), then it counts itself done.
+    const int h = __builtin_amdgcn_readfirstlane(desc.aux);
+    double *slot = a.heavy_part + 16 * (size_t)(__builtin_amdgcn_readfirstlane(a.heavy_base[h]) + (flags >> 8));
+    wave_spmv<MODE, COUNT, 3, false, MODE != kAll64>(a.A, desc, x, x32, seglev, force64, L, lane, cnt,
+      [&](const int i, const double v) {
+        __hip_atomic_store(slot + (i & 15), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      });
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) __hip_atomic_fetch_add(a.heavy_cnt + h, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // The heavy tile-rows the wave owns: once every piece has counted itself
+  // done, the owner adds the pieces' partial sums in piece order and runs the
+  // epilogue, so the rows' values and their terms in the wave's dot-product
+  // partials are the same on every run.  Pieces never wait, and a wave waits
+  // only after its own descriptors (its pieces included), so the wait ends.
+  const int h0 = __builtin_amdgcn_readfirstlane(a.wave_heavy[wid]);
+  const int h1 = __builtin_amdgcn_readfirstlane(a.wave_heavy[wid + 1]);
+  for (int h = h0; h < h1; ++h) {
+    const unsigned np = (unsigned)__builtin_amdgcn_readfirstlane(a.heavy_n[h]);
+    unsigned spins = 0;
+    while (__hip_atomic_load(a.heavy_cnt + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < np) {
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(&a.gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      if ((spins & 1023u) == 0 &&
+          __hip_atomic_load(&a.gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int base = __builtin_amdgcn_readfirstlane(a.heavy_base[h]);
+    // lane (q, row): pieces q, q+4, ... of the row, then the four partial
+    // sums in q order
+    const int row = lane & 15, q = lane >> 4;
+    double sum = 0.0;
+    for (int k = q; k < (int)np; k += 4)
+      sum += __hip_atomic_load(a.heavy_part + 16 * (size_t)(base + k) + row, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    const double s1 = __shfl(sum, row + 16), s2 = __shfl(sum, row + 32), s3 = __shfl(sum, row + 48);
+    const int R = __builtin_amdgcn_readfirstlane(a.heavy_tr[h]);
+    if (lane < 16) epi(16 * R + lane, ((sum + s1) + s2) + s3);
+    // reset for the next SpMV (a grid barrier separates it from this one)
+    if (lane == 0) __hip_atomic_store(a.heavy_cnt + h, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Learned levels for exactly the tiles the calling wave's SpMV reads: its
+// resident window, or its descriptors (whole tile-rows, and the pieces of
+// heavy tile-rows it holds; a heavy tile-row's tiles are inferred by the
+// waves that multiply them, never by its owner).
+template <bool RES>
+__device__ __forceinline__ void solver_infer(const SolveArgs &a, const InferCtx &ic, int tr0, int tr1,
+                                             int lane) {
+  if constexpr (RES) {
+    wave_infer(a.A, ic, tr0, tr1, lane);
+  } else {
+    const int wid = solver_wave_id();
+    const int d0 = __builtin_amdgcn_readfirstlane(a.wave_win[wid]);
+    const int d1 = __builtin_amdgcn_readfirstlane(a.wave_win[wid + 1]);
+    for (int d = d0; d < d1; ++d)
+      wave_infer_tiles(a.A, ic, __builtin_amdgcn_readfirstlane(a.win_desc[d].ta),
+                       __builtin_amdgcn_readfirstlane(a.win_desc[d].T1), lane);
+  }
 }
 
 // Dot products whose next phase reads only the wave's own rows.  With
@@ -354,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       // w = A r with gamma = r.r and delta = w.r of the wave's rows; new
       // levels (inference, or a rule update before the last sync) are staged
       // into a resident window first
-      if (infer && !force64) wave_infer(a.A, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
+      if (infer && !force64) solver_infer<RES>(a, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
       if ((infer && !force64) || restage) solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
       infer = false;
       restage = false;
@@ -641,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     // v = A p, v.rh
     double vr[1] = {0.0};
     if (infer && !force64) {
-      wave_infer(a.A, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
+      solver_infer<RES>(a, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
       solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
     }
     infer = false;
@@ -1195,7 +1266,7 @@ mf_status grid_for(K kernel, int override_blocks, int *G, int block = kBlock, bo
 // Host side of a solver schedule: per-wave tile-row bounds, resident flags,
 // the waves' SpMV descriptors and the heavy tile-row pieces.
 struct HostSchedule {
-  std::vector<int> b, res, ww, hn, hbase;
+  std::vector<int> b, res, ww, hn, hbase, htr, wh;
   std::vector<WaveDesc> wd;
   int npieces = 0, all_resident = 1;
 };
@@ -1267,7 +1338,18 @@ void build_schedule(const mf_tiled *t, int nwaves, bool even_rows, HostSchedule 
     }
     hn.push_back(np);
     hbase.push_back(npieces);
+    hsch.htr.push_back(r);
     npieces += np;
+  }
+  // heavy tile-rows each wave owns (they are numbered in tile-row order)
+  hsch.wh.assign(nwaves + 1, 0);
+  {
+    size_t h = 0;
+    for (int w = 0; w < nwaves; ++w) {
+      hsch.wh[w] = (int)h;
+      while (h < hsch.htr.size() && hsch.htr[h] < b[w + 1]) ++h;
+    }
+    hsch.wh[nwaves] = (int)h;
   }
   std::vector<int> ww(nwaves + 1, 0);
   std::vector<WaveDesc> wd;
@@ -1306,7 +1388,7 @@ mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
   if (t->sched.wave_tr && t->sched_waves_for == key) return MF_OK;
   for (void *p : {(void *)t->sched.wave_tr, (void *)t->sched.wave_res, (void *)t->sched.wave_win, (void *)t->sched.win_desc,
                   (void *)t->sched.heavy_part, (void *)t->sched.heavy_cnt, (void *)t->sched.heavy_n,
-                  (void *)t->sched.heavy_base})
+                  (void *)t->sched.heavy_base, (void *)t->sched.heavy_tr, (void *)t->sched.wave_heavy})
     if (p) MF_HIP(hipFree(p));
   t->sched = Schedule();
   t->sched_waves_for = -1;
@@ -1329,9 +1411,13 @@ mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
   MF_HIP(hipMalloc(&t->sched.heavy_cnt, sizeof(unsigned) * std::max(nh, 1)));
   MF_HIP(hipMalloc(&t->sched.heavy_n, sizeof(int) * std::max(nh, 1)));
   MF_HIP(hipMalloc(&t->sched.heavy_base, sizeof(int) * std::max(nh, 1)));
+  MF_HIP(hipMalloc(&t->sched.heavy_tr, sizeof(int) * std::max(nh, 1)));
+  MF_HIP(hipMalloc(&t->sched.wave_heavy, sizeof(int) * (nwaves + 1)));
+  MF_HIP(hipMemcpy(t->sched.wave_heavy, hsch.wh.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice));
   if (nh > 0) {
     MF_HIP(hipMemcpy(t->sched.heavy_n, hn.data(), sizeof(int) * nh, hipMemcpyHostToDevice));
     MF_HIP(hipMemcpy(t->sched.heavy_base, hbase.data(), sizeof(int) * nh, hipMemcpyHostToDevice));
+    MF_HIP(hipMemcpy(t->sched.heavy_tr, hsch.htr.data(), sizeof(int) * nh, hipMemcpyHostToDevice));
   }
   t->sched.nheavy = nh;
   t->sched.nwaves = nwaves;
@@ -1410,6 +1496,7 @@ mf_status scratch_layout(const mf_tiled *t, int G, const HostSchedule &hs, unsig
   const size_t o_hpart = take(sizeof(double) * 16 * std::max(hs.npieces, 1));
   const size_t o_hcnt = take(sizeof(unsigned) * std::max(nh, 1)), o_hn = take(sizeof(int) * std::max(nh, 1));
   const size_t o_hbase = take(sizeof(int) * std::max(nh, 1));
+  const size_t o_htr = take(sizeof(int) * std::max(nh, 1)), o_wh = take(sizeof(int) * (nwaves + 1));
   if (off > kScratchBytes)
     return fail(MF_ERR_ARG, "mf_cg_solve_noalloc: the system needs " + std::to_string(off) +
                                 " bytes of workspace, more than the " + std::to_string(kScratchBytes) +
@@ -1445,6 +1532,9 @@ mf_status scratch_layout(const mf_tiled *t, int G, const HostSchedule &hs, unsig
   sc.heavy_cnt = (unsigned *)(base + o_hcnt);
   sc.heavy_n = (int *)(base + o_hn);
   sc.heavy_base = (int *)(base + o_hbase);
+  sc.heavy_tr = (int *)(base + o_htr);
+  sc.wave_heavy = (int *)(base + o_wh);
+  MF_HIP(hipMemcpyAsync(sc.wave_heavy, hs.wh.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice, st));
   MF_HIP(hipMemcpyAsync(sc.wave_tr, hs.b.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice, st));
   MF_HIP(hipMemcpyAsync(sc.wave_res, hs.res.data(), sizeof(int) * nwaves, hipMemcpyHostToDevice, st));
   MF_HIP(hipMemcpyAsync(sc.wave_win, hs.ww.data(), sizeof(int) * (nwaves + 1), hipMemcpyHostToDevice, st));
@@ -1452,6 +1542,7 @@ mf_status scratch_layout(const mf_tiled *t, int G, const HostSchedule &hs, unsig
   if (nh > 0) {
     MF_HIP(hipMemcpyAsync(sc.heavy_n, hs.hn.data(), sizeof(int) * nh, hipMemcpyHostToDevice, st));
     MF_HIP(hipMemcpyAsync(sc.heavy_base, hs.hbase.data(), sizeof(int) * nh, hipMemcpyHostToDevice, st));
+    MF_HIP(hipMemcpyAsync(sc.heavy_tr, hs.htr.data(), sizeof(int) * nh, hipMemcpyHostToDevice, st));
   }
   // the scratch keeps the last call's contents: clear what starts at zero
   MF_TRY(fill_u32(base + o_vec, o_segp - o_vec, 0u, st));  // vectors (zero padding)
@@ -1608,6 +1699,8 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   a.heavy_cnt = sch->heavy_cnt;
   a.heavy_n = sch->heavy_n;
   a.heavy_base = sch->heavy_base;
+  a.heavy_tr = sch->heavy_tr;
+  a.wave_heavy = sch->wave_heavy;
   a.partials = w.partials;
   a.mbox = w.mbox;
   a.gs = w.gs;

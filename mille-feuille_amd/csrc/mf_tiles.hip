@@ -261,7 +261,8 @@ void free_device(mf_tiled *t) {
                   d.v64,      d.v32,      d.v16,      d.v8,       d.s16,    d.s8,
                   d.qerr,     d.amax,     d.prec,     d.prec_user, d.diag_tile, d.mlp, d.feat16,
                   t->sched.wave_tr, t->sched.wave_res, t->sched.wave_win, t->sched.win_desc, t->sched.heavy_part,
-                  t->sched.heavy_cnt, t->sched.heavy_n, t->sched.heavy_base, t->work};
+                  t->sched.heavy_cnt, t->sched.heavy_n, t->sched.heavy_base, t->sched.heavy_tr,
+                  t->sched.wave_heavy, t->work};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   d = DeviceLayout();

@@ -123,6 +123,38 @@ def test_solver_is_deterministic_and_takes_initial_guess(mf, oracle):
     assert oracle.rel_residual(rp, ci, va, b, dx.get()[:n]) <= TOL
 
 
+def _tiles_per_tile_row(rp, ci):
+    n = len(rp) - 1
+    tr = np.repeat(np.arange(n) // 16, np.diff(rp)).astype(np.int64)
+    key = np.unique(tr * (n // 16 + 1) + ci // 16)
+    return np.bincount(key // (n // 16 + 1), minlength=(n + 15) // 16)
+
+
+@pytest.mark.parametrize("method,policy", [("cg", "learned"), ("bicgstab", "learned"), ("cg", "fp64")])
+def test_heavy_tile_rows_are_deterministic(mf, oracle, method, policy):
+    """A power-law matrix whose tile-rows mostly hold more tiles than one LDS
+    window (128): they are cut into pieces spread over the waves.  The owner
+    of each such tile-row adds the pieces' partial sums in piece order, and
+    each wave infers the levels of exactly the tiles it multiplies, so two
+    runs give bit-identical solutions."""
+    rp, ci, va, b = _system("powerlaw")
+    n = len(rp) - 1
+    assert (_tiles_per_tile_row(rp, ci) > 128).sum() > 100
+    tm = mf.TiledMatrix(rp, ci, va)
+    db = tm.vector(b)
+    xs, its = [], []
+    for _ in range(2):
+        print(f"case powerlaw {method} {policy} run {len(xs)}", flush=True)
+        dx = tm.vector()
+        rep = getattr(tm, method)(db, dx, tol=TOL, policy=policy)
+        assert rep["status"] == 0, rep
+        xs.append(dx.get())
+        its.append(rep["iterations"])
+    assert its[0] == its[1]
+    np.testing.assert_array_equal(xs[0], xs[1])
+    assert oracle.rel_residual(rp, ci, va, b, xs[0][:n]) <= TOL
+
+
 @pytest.mark.parametrize("policy", ["fp64", "errbound"])
 def test_resident_form_is_deterministic(mf, oracle, policy):
     rp, ci, va, b = _system("lap3d_scaled32")
