@@ -11,12 +11,30 @@
 // (src/main-cg.cu:1012-1022, src/main-bicg.cu:676-688; a missing data/ folder
 // prints "open error!" and returns, as there).  For BiCGSTAB, rhs is
 // overwritten with the row sums of A (src/main-bicg.cu:480-487).
+// CSV fields 4 and 5 are the reference's: for CG the FP64 check and the
+// absolute recursive residual norm sqrt(r.r) (src/main-cg.cu:1013); for
+// BiCGSTAB the recursive ||r|| / ||rhs|| and ||r||^2 / ||r0||
+// (src/main-bicg.cu:675-678, where k_residual accumulates r.r and err_rel is
+// ||r0||).
+//
+// The same names mean FP64 solvers in main-cg / main-bicg and mixed ones in
+// main-cg-mixed / main-bicg-mixed (src/main-cg-mixed.cu:2238,
+// src/main-bicg-mixed.cu:1355).  This file is therefore built twice: into
+// libmillefeuille_hip.so, whose entry points default to FP64 (cg_solve_reduce
+// and bicgstab to the multi-kernel solvers, as src/main-cg.cu:6 and
+// src/main-bicg.cu:1336), and with MF_COMPAT_MIXED into
+// libmillefeuille_mixed.so, whose entry points all default to the
+// single-kernel solvers under the reference's rule policy.
 // Differences: the solve stops at the FP64 relative residual 1e-10 instead of
 // running a fixed 100 (CG) / 50 (BiCGSTAB) iterations; MF_COMPAT_FIXED_ITERS=k
 // in the environment restores a fixed count.  MF_COMPAT_POLICY selects the
-// mixed policy (fp64, rule, errbound, fixed, learned); the *_mixed entry
-// points default to the reference's rule policy.  The learned policy loads
+// policy of every entry point (fp64, rule, errbound, fixed, learned); a policy
+// other than fp64 always runs the single-kernel solver.  The *_mixed entry
+// points default to the rule policy in both builds.  The learned policy loads
 // its MLP from MF_POLICY_FILE, else from policy_mlp.f32 next to this library.
+// One line per solve on stdout names the solver that ran: "mf: cg
+// single-kernel policy=rule iterations=N levels=a/b/c/d/e", the last field
+// being the tile-SpMV counts at FP64/FP32/FP16/INT8/skipped.
 #include <dlfcn.h>
 
 #include <cmath>
@@ -29,6 +47,12 @@
 #include "mf_device.h"
 
 namespace {
+
+#ifdef MF_COMPAT_MIXED
+constexpr int kDefaultPolicy = MF_POLICY_RULE;
+#else
+constexpr int kDefaultPolicy = MF_POLICY_FP64;
+#endif
 
 int env_int(const char *name, int dflt) {
   const char *v = std::getenv(name);
@@ -45,6 +69,30 @@ int env_policy(int dflt) {
   if (s == "errbound") return MF_POLICY_ERRBOUND;
   if (s == "learned") return MF_POLICY_LEARNED;
   return dflt;
+}
+
+const char *policy_name(int p) {
+  switch (p) {
+    case MF_POLICY_FP64: return "fp64";
+    case MF_POLICY_RULE: return "rule";
+    case MF_POLICY_FIXED: return "fixed";
+    case MF_POLICY_ERRBOUND: return "errbound";
+    case MF_POLICY_LEARNED: return "learned";
+    default: return "?";
+  }
+}
+
+void print_solver(const char *method, bool multikernel, const mf_report &rep) {
+  std::printf("mf: %s %s policy=%s iterations=%d levels=%lld/%lld/%lld/%lld/%lld\n", method,
+              multikernel ? "multi-kernel" : "single-kernel", policy_name(rep.policy_used),
+              rep.iterations, rep.prec_hist[0], rep.prec_hist[1], rep.prec_hist[2], rep.prec_hist[3],
+              rep.prec_hist[4]);
+}
+
+double norm2(const double *v, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += v[i] * v[i];
+  return std::sqrt(s);
 }
 
 // Installs the shipped MLP (1476 little-endian float32) on the handle;
@@ -124,6 +172,7 @@ struct DevVec {
 // Shared CG driver: rows [0, rowA), solve on device 0, x (n entries) out.
 void compat_cg(int *RowPtr, int *ColIdx, double *Val, float *Val_Low, double *x, double *b,
                int n, char *filename, int nnzR, int ori, int policy, bool multikernel) {
+  multikernel = multikernel && policy == MF_POLICY_FP64;
   const int rowA = (n / 16) * 16;
   std::memset(x, 0, sizeof(double) * n);
   mf_tiled *t = nullptr;
@@ -156,17 +205,19 @@ void compat_cg(int *RowPtr, int *ColIdx, double *Val, float *Val_Low, double *x,
   if (st < 0) std::printf("mf: %s\n", mf_last_error());
   (void)hipMemcpy(hx.data(), dx.p, sizeof(double) * npad, hipMemcpyDeviceToHost);
   for (int i = 0; i < n; ++i) x[i] = i < rowA ? hx[i] : 0.0;
+  print_solver("cg", multikernel, rep);
   std::printf("time_cg=%lf ms\n", rep.time_ms);
   const double l2 = tile_check(t, x, b, rowA);
   char row[256];
   std::snprintf(row, sizeof row, "%d,%.3f,%d,%e,%e\n", rep.iterations, rep.time_ms, nnzR, l2,
-                rep.rel_residual_recursive);
+                rep.rel_residual_recursive * norm2(b, rowA));
   append_csv("data/cg_performance.csv", filename, row);
   mf_tile_free(t);
 }
 
 int compat_bicg(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, int nnzR,
                 char *filename, int n, int policy, bool multikernel) {
+  multikernel = multikernel && policy == MF_POLICY_FP64;
   const int rowA = (n / 16) * 16;
   for (int i = 0; i < n; ++i) {
     double s = 0.0;
@@ -203,11 +254,13 @@ int compat_bicg(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, i
   if (st < 0) std::printf("mf: %s\n", mf_last_error());
   (void)hipMemcpy(hx.data(), dx.p, sizeof(double) * npad, hipMemcpyDeviceToHost);
   for (int i = 0; i < n; ++i) x[i] = i < rowA ? hx[i] : 0.0;
+  print_solver("bicgstab", multikernel, rep);
   std::printf("time_bicg=%lf ms\n", rep.time_ms);
-  const double l2 = tile_check(t, x, rhs, rowA);
+  // ||r|| / ||rhs|| and ||r||^2 / ||r0|| with r0 = rhs (x starts at zero)
+  const double rel = rep.rel_residual_recursive, bnorm = norm2(rhs, rowA);
   char row[256];
-  std::snprintf(row, sizeof row, "%d,%.3f,%d,%e,%e\n", rep.iterations, rep.time_ms, nnzR, l2,
-                rep.rel_residual_recursive);
+  std::snprintf(row, sizeof row, "%d,%.3f,%d,%e,%e\n", rep.iterations, rep.time_ms, nnzR, rel,
+                rel * rel * bnorm);
   append_csv("data/bicg_performance.csv", filename, row);
   mf_tile_free(t);
   return st == MF_OK ? 0 : 1;
@@ -219,21 +272,21 @@ extern "C" void cg_solve_reduce(int *RowPtr, int *ColIdx, double *Val, float *Va
                                 double *b, int n, int *iter, int maxiter, double threshold,
                                 char *filename, int nnzR, int ori) {
   (void)iter; (void)maxiter; (void)threshold;
-  compat_cg(RowPtr, ColIdx, Val, Val_Low, x, b, n, filename, nnzR, ori, MF_POLICY_FP64, true);
+  compat_cg(RowPtr, ColIdx, Val, Val_Low, x, b, n, filename, nnzR, ori, env_policy(kDefaultPolicy), true);
 }
 
 extern "C" void cg_solve_sync(int *RowPtr, int *ColIdx, double *Val, float *Val_Low, double *x,
                               double *b, int n, int *iter, int maxiter, double threshold,
                               char *filename, int nnzR, int ori) {
   (void)iter; (void)maxiter; (void)threshold;
-  compat_cg(RowPtr, ColIdx, Val, Val_Low, x, b, n, filename, nnzR, ori, env_policy(MF_POLICY_FP64), false);
+  compat_cg(RowPtr, ColIdx, Val, Val_Low, x, b, n, filename, nnzR, ori, env_policy(kDefaultPolicy), false);
 }
 
 extern "C" void cg_solve_inc(int *RowPtr, int *ColIdx, double *Val, float *Val_Low, double *x,
                              double *b, int n, int *iter, int maxiter, double threshold,
                              char *filename, int nnzR, int ori) {
   (void)iter; (void)maxiter; (void)threshold;
-  compat_cg(RowPtr, ColIdx, Val, Val_Low, x, b, n, filename, nnzR, ori, env_policy(MF_POLICY_FP64), false);
+  compat_cg(RowPtr, ColIdx, Val, Val_Low, x, b, n, filename, nnzR, ori, env_policy(kDefaultPolicy), false);
 }
 
 extern "C" void cg_solve_mixed(int *RowPtr, int *ColIdx, double *Val, float *Val_Low, double *x,
@@ -245,17 +298,17 @@ extern "C" void cg_solve_mixed(int *RowPtr, int *ColIdx, double *Val, float *Val
 
 extern "C" int bicgstab(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x, int nnzR,
                         char *filename, int n) {
-  return compat_bicg(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, MF_POLICY_FP64, true);
+  return compat_bicg(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, env_policy(kDefaultPolicy), true);
 }
 
 extern "C" int bicgstab_sync(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x,
                              int nnzR, char *filename, int n) {
-  return compat_bicg(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, env_policy(MF_POLICY_FP64), false);
+  return compat_bicg(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, env_policy(kDefaultPolicy), false);
 }
 
 extern "C" int bicgstab_inc(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x,
                             int nnzR, char *filename, int n) {
-  return compat_bicg(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, env_policy(MF_POLICY_FP64), false);
+  return compat_bicg(RowPtr, ColIdx, Val, rhs, x, nnzR, filename, n, env_policy(kDefaultPolicy), false);
 }
 
 extern "C" int bicgstab_mixed(int *RowPtr, int *ColIdx, double *Val, double *rhs, double *x,

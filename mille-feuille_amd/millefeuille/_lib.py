@@ -11,6 +11,9 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmillefeuille_hip.so")
+# the reference-named entry points with mixed-precision defaults (the drop-in
+# for the mixed executables); loaded on demand by its users
+MIXED_LIB_PATH = os.path.join(_HERE, "libmillefeuille_mixed.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(

@@ -225,6 +225,98 @@ def test_reference_compatible_entry_points(mf, oracle, tmp_path, monkeypatch):
     assert (tmp_path / "data" / "bicg_performance.csv").exists()
 
 
+def _libc_flush():
+    import ctypes as C
+    C.CDLL(None).fflush(None)
+
+
+def test_mixed_executables_reach_the_mixed_single_kernel_solvers(mf, oracle, tmp_path, monkeypatch,
+                                                                 capfd):
+    """The calls of the two mixed executables, made through
+    libmillefeuille_mixed.so with their argument lists: cg_solve_inc as
+    src/main-cg-mixed.cu:3006-3041 (n truncated to a multiple of 16, X = 1 on
+    entry, b = A * 1) and bicgstab as src/main-bicg-mixed.cu:2168-2188
+    (rhs = 1 and x = 0 on entry, rhs overwritten with A * 1).  stdout names the
+    single-kernel solver under the rule policy, the CSV rows carry the
+    reference's fields, and the FP64 residual is <= 1e-10."""
+    import ctypes as C
+    from millefeuille import gen
+    from millefeuille._lib import MIXED_LIB_PATH
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.delenv("MF_COMPAT_POLICY", raising=False)
+    monkeypatch.delenv("MF_COMPAT_FIXED_ITERS", raising=False)
+    (tmp_path / "data").mkdir()
+    lib = C.CDLL(MIXED_LIB_PATH)
+    p = lambda a: a.ctypes.data  # noqa: E731
+
+    rp, ci, va = gen.laplace3d_scaled(32, seed=2)
+    ori = len(rp) - 1
+    n = (ori // 16) * 16
+    low = va.astype(np.float32)
+    x = np.ones(n)
+    b = gen.csr_matvec(rp, ci, va, np.ones(ori))[:n].copy()
+    it = C.c_int(0)
+    print("case mixed cg_solve_inc lap3d_scaled32", flush=True)
+    lib.cg_solve_inc(C.c_void_p(p(rp)), C.c_void_p(p(ci)), C.c_void_p(p(va)), C.c_void_p(p(low)),
+                     C.c_void_p(p(x)), C.c_void_p(p(b)), n, C.byref(it), 10, C.c_double(1e-5),
+                     b"lap3d.mtx", int(rp[-1]), ori)
+    _libc_flush()
+    out = capfd.readouterr().out
+    line = [ln for ln in out.splitlines() if ln.startswith("mf: cg ")]
+    assert line and "single-kernel policy=rule" in line[0], out
+    levels = [int(v) for v in line[0].split("levels=")[1].split("/")]
+    assert sum(levels[1:4]) > 0, line[0]  # FP32/FP16/INT8 tiles ran
+    assert oracle.rel_residual(rp, ci, va, b, x) <= TOL
+    row = (tmp_path / "data" / "cg_performance.csv").read_text().strip().split(",")
+    assert row[0] == "lap3d.mtx" and int(row[3]) == rp[-1] and float(row[4]) <= TOL
+    # field 5: sqrt(r.r), the absolute recursive residual norm
+    assert 0.0 < float(row[5]) <= 2 * TOL * np.linalg.norm(b), row
+
+    rp2, ci2, va2 = gen.convdiff3d(20)
+    m = len(rp2) - 1
+    rhs, x2 = np.ones(m), np.zeros(m)
+    print("case mixed bicgstab convdiff20", flush=True)
+    ret = lib.bicgstab(C.c_void_p(p(rp2)), C.c_void_p(p(ci2)), C.c_void_p(p(va2)), C.c_void_p(p(rhs)),
+                       C.c_void_p(p(x2)), int(rp2[-1]), b"cd.mtx", m)
+    _libc_flush()
+    out = capfd.readouterr().out
+    assert ret == 0, out
+    line = [ln for ln in out.splitlines() if ln.startswith("mf: bicgstab ")]
+    assert line and "single-kernel policy=rule" in line[0], out
+    np.testing.assert_allclose(rhs, np.add.reduceat(va2, rp2[:-1]))
+    assert oracle.rel_residual(rp2, ci2, va2, rhs, x2) <= TOL
+    row = (tmp_path / "data" / "bicg_performance.csv").read_text().strip().split(",")
+    rel, r2 = float(row[4]), float(row[5])
+    # fields 4 and 5: ||r|| / ||rhs|| and ||r||^2 / ||r0||, r0 = rhs
+    assert 0.0 < rel <= 2 * TOL
+    np.testing.assert_allclose(r2, rel * rel * np.linalg.norm(rhs), rtol=1e-5)
+
+
+def test_compat_policy_switch_reaches_bicgstab(mf, oracle, tmp_path, monkeypatch, capfd):
+    """In the FP64 library, bicgstab is the multi-kernel FP64 solver unless
+    MF_COMPAT_POLICY asks for a mixed policy."""
+    import ctypes as C
+    from millefeuille import gen
+    from millefeuille._lib import lib
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "data").mkdir()
+    rp, ci, va = gen.convdiff3d(16)
+    m = len(rp) - 1
+    p = lambda a: a.ctypes.data  # noqa: E731
+    for env, want in ((None, "multi-kernel policy=fp64"), ("errbound", "single-kernel policy=errbound")):
+        if env is None:
+            monkeypatch.delenv("MF_COMPAT_POLICY", raising=False)
+        else:
+            monkeypatch.setenv("MF_COMPAT_POLICY", env)
+        rhs, x = np.ones(m), np.zeros(m)
+        print(f"case bicgstab MF_COMPAT_POLICY={env}", flush=True)
+        assert lib.bicgstab(p(rp), p(ci), p(va), p(rhs), p(x), int(rp[-1]), b"cd.mtx", m) == 0
+        _libc_flush()
+        out = capfd.readouterr().out
+        assert any(ln.startswith("mf: bicgstab " + want) for ln in out.splitlines()), out
+        assert oracle.rel_residual(rp, ci, va, rhs, x) <= TOL
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_driver_cli(mixed, tmp_path):
     import subprocess

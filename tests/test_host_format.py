@@ -181,6 +181,24 @@ def test_abi_exports_every_declared_symbol():
     assert decl <= set(SIGNATURES), sorted(decl - set(SIGNATURES))
 
 
+def test_mixed_compat_library_exports_the_reference_names():
+    """libmillefeuille_mixed.so (mixed-precision defaults, for the mixed
+    executables) loads beside the main library and defines every
+    reference-named solver entry point itself."""
+    import ctypes as C
+    import subprocess
+    from millefeuille._lib import MIXED_LIB_PATH
+    names = ["cg_solve_reduce", "cg_solve_sync", "cg_solve_inc", "cg_solve_mixed", "bicgstab",
+             "bicgstab_sync", "bicgstab_inc", "bicgstab_mixed"]
+    lib = C.CDLL(MIXED_LIB_PATH)
+    for nm in names:
+        assert hasattr(lib, nm), nm
+    out = subprocess.run(["nm", "-D", "--defined-only", MIXED_LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    defined = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert set(names) <= defined, sorted(set(names) - defined)
+
+
 def test_product_does_not_reference_the_oracle():
     """The shipped library and package never load or call the oracle."""
     pkg = os.path.join(ROOT, "mille-feuille_amd")
