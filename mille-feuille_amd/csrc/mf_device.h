@@ -477,7 +477,7 @@ struct LvGroups {
 template <int LV>
 struct LvChunk {
   uint32_t rc4;
-  int vmask, tcol, ro;
+  int valid, tcol, ro;    // valid: the lane's position lies in the group
   float sc;
   u32x4_t a, b;           // payload words (b: FP64 only)
   double d0, d1, d2, d3;  // x of FP64 tiles multiplying FP64 x
@@ -599,8 +599,11 @@ __device__ __forceinline__ WinPay win_pay(const TileView &A, int Z0) {
 }
 
 // Chunk at group position pos (< end; a lane past the end repeats the last
-// chunk with no valid slot, so every slice issues the same loads): LDS
+// chunk and is marked invalid, so every slice issues the same loads): LDS
 // lookups, then the (row,col) bytes and the payload at the level's width.
+// A chunk needs no per-slot mask: the padding slots of a tile hold value 0
+// and repeat its last (row,col) byte, so they add exact zeros to the row the
+// chunk's last real slot adds to.
 template <int LV, class LDS>
 __device__ __forceinline__ void lv_load(const WinPay &W, const LDS &L, int pos, int end,
                                         LvChunk<LV> &s) {
@@ -609,8 +612,7 @@ __device__ __forceinline__ void lv_load(const WinPay &W, const LDS &L, int pos, 
   const int j = L.ctile[p];
   const int te = L.tend[j];
   const int c = p + (te >> 13) - 1024;
-  const int live = (te & 8191) - 4 * c;
-  s.vmask = valid ? (live >= 4 ? 15 : (1 << live) - 1) : 0;
+  s.valid = valid;
   s.tcol = L.tcol[j];
   s.ro = L.trow[j] * 16;
   s.sc = (LV == MF_PREC_FP16 || LV == MF_PREC_INT8) ? L.tscale[j] : 1.0f;
@@ -650,52 +652,54 @@ __device__ __forceinline__ void lv_gather(const double *__restrict__ x, const fl
   }
 }
 
+// Slot u's product: FP64 for FP64 tiles, FP32 for the others (the FP16 and
+// INT8 tile scale applied once per slot).
+template <int LV>
+using LvAcc = typename std::conditional<LV == MF_PREC_FP64, double, float>::type;
+
 template <int LV, bool X32>
-__device__ __forceinline__ double lv_product(const LvChunk<LV> &s, int u) {
+__device__ __forceinline__ LvAcc<LV> lv_product(const LvChunk<LV> &s, int u) {
   const float xf = u == 0 ? s.f0 : u == 1 ? s.f1 : u == 2 ? s.f2 : s.f3;
-  if (LV == MF_PREC_FP64) {
+  if constexpr (LV == MF_PREC_FP64) {
     const uint32_t lo = u == 0 ? s.a.x : u == 1 ? s.a.z : u == 2 ? s.b.x : s.b.z;
     const uint32_t hi = u == 0 ? s.a.y : u == 1 ? s.a.w : u == 2 ? s.b.y : s.b.w;
     const double a = __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
     const double xd = u == 0 ? s.d0 : u == 1 ? s.d1 : u == 2 ? s.d2 : s.d3;
     return a * (X32 ? (double)xf : xd);
-  } else if (LV == MF_PREC_FP32) {
+  } else if constexpr (LV == MF_PREC_FP32) {
     const uint32_t w = u == 0 ? s.a.x : u == 1 ? s.a.y : u == 2 ? s.a.z : s.a.w;
-    return (double)(__builtin_bit_cast(float, w) * xf);
-  } else if (LV == MF_PREC_FP16) {
+    return __builtin_bit_cast(float, w) * xf;
+  } else if constexpr (LV == MF_PREC_FP16) {
     const uint32_t w = u < 2 ? s.a.x : s.a.y;
-    return (double)(half_bits_to_float((uint16_t)(w >> (16 * (u & 1)))) * xf * s.sc);
+    return half_bits_to_float((uint16_t)(w >> (16 * (u & 1)))) * xf * s.sc;
   } else {
-    return (double)((float)(int8_t)(s.a.x >> (8 * u)) * xf * s.sc);
+    return (float)(int8_t)(s.a.x >> (8 * u)) * xf * s.sc;
   }
 }
 
-// Adds four slot products into row accumulators k0..k3 (see chunk_accumulate).
-template <class LDS>
-__device__ __forceinline__ void lv_accumulate(LDS &L, int k0, int k1, int k2, int k3, double p0,
-                                              double p1, double p2, double p3) {
-  const double s1 = p1 + (k1 == k0 ? p0 : 0.0);
-  const double s2 = p2 + (k2 == k1 ? s1 : 0.0);
-  const double s3 = p3 + (k3 == k2 ? s2 : 0.0);
-  if (k1 != k0) atomicAdd(&L.acc[k0], p0);
-  if (k2 != k1) atomicAdd(&L.acc[k1], s1);
-  if (k3 != k2) atomicAdd(&L.acc[k2], s2);
-  atomicAdd(&L.acc[k3], s3);
+// Adds four slot products into row accumulators k0..k3 (see
+// chunk_accumulate): runs of equal rows are summed in T (FP32 below FP64)
+// and each run is added to its row once, in FP64.
+template <class LDS, typename T>
+__device__ __forceinline__ void lv_accumulate(LDS &L, int k0, int k1, int k2, int k3, T p0, T p1,
+                                              T p2, T p3) {
+  const T s1 = p1 + (k1 == k0 ? p0 : T(0));
+  const T s2 = p2 + (k2 == k1 ? s1 : T(0));
+  const T s3 = p3 + (k3 == k2 ? s2 : T(0));
+  if (k1 != k0) atomicAdd(&L.acc[k0], (double)p0);
+  if (k2 != k1) atomicAdd(&L.acc[k1], (double)s1);
+  if (k3 != k2) atomicAdd(&L.acc[k2], (double)s2);
+  atomicAdd(&L.acc[k3], (double)s3);
 }
 
-// Products of the chunk's valid slots into the window's row accumulators.
-// Every slot's product is computed and then selected (no branch per slot, so
-// no wait inside a branch).
+// Products of the chunk's slots into the window's row accumulators (nothing
+// for a lane past the group's end).
 template <int LV, bool X32, class LDS>
 __device__ __forceinline__ void lv_consume(LDS &L, const LvChunk<LV> &s) {
-  if (!s.vmask) return;
-  double p[4];
+  if (!s.valid) return;
+  LvAcc<LV> p[4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    double v = lv_product<LV, X32>(s, u);
-    asm volatile("" : "+v"(v));
-    p[u] = (s.vmask >> u & 1) ? v : 0.0;
-  }
+  for (int u = 0; u < 4; ++u) p[u] = lv_product<LV, X32>(s, u);
   const int r0 = s.ro + ((s.rc4 >> 4) & 15), r1 = s.ro + ((s.rc4 >> 12) & 15);
   const int r2 = s.ro + ((s.rc4 >> 20) & 15), r3 = s.ro + ((s.rc4 >> 28) & 15);
   lv_accumulate(L, r0, r1, r2, r3, p[0], p[1], p[2], p[3]);
@@ -706,23 +710,36 @@ __device__ __forceinline__ void lv_consume(LDS &L, const LvChunk<LV> &s) {
 // k+D-1 are in flight.  Each step issues slice k+D-1's loads before slice
 // k+1's gathers, so waiting for slice k+1's (row,col) bytes (loaded D - 2
 // steps earlier) never waits for slice k's gathers (loads retire in issue
-// order), and a step waits on no load younger than one step.  The loop body
-// issues the same loads on every trip (slices past the group repeat its last
-// chunk with no valid slot), so the number of loads in flight at every wait
-// is fixed.  FP64 chunks with FP64 x hold the most registers and run three
-// slices deep, the others four.
+// order), and a step waits on no load younger than one step.  Every step
+// issues the same loads (slices past the group repeat its last chunk,
+// marked invalid), so the number of loads in flight at every wait is fixed;
+// the steps are those of whole trips of the D-step body, then a tail of the
+// remaining nsl mod D steps.  FP64 chunks with FP64 x hold the most
+// registers and run three slices deep, the others four.
 template <int LV, bool X32>
 struct LvDepth {
   static constexpr int D = (LV == MF_PREC_FP64 && !X32) ? 3 : 4;
 };
+
+// Step i of a trip starting at slice k (c[] rotates with i, so i is static).
+template <int LV, bool X32, int D, int I, class LDS>
+__device__ __forceinline__ void level_step(const WinPay &W, LDS &L, int q, int e,
+                                           const double *__restrict__ x,
+                                           const float *__restrict__ x32, LvChunk<LV> (&c)[D]) {
+  lv_load<LV>(W, L, q + 64 * I, e, c[(I + D - 1) % D]);
+  __builtin_amdgcn_sched_barrier(0);
+  lv_gather<LV, X32>(x, x32, c[(I + 1) % D]);
+  __builtin_amdgcn_sched_barrier(0);
+  lv_consume<LV, X32>(L, c[I]);
+}
 
 template <int LV, bool X32, class LDS>
 __device__ __forceinline__ void level_group(const TileView &A, LDS &L, int Z0, int b, int e,
                                             const double *__restrict__ x,
                                             const float *__restrict__ x32, int lane) {
   constexpr int D = LvDepth<LV, X32>::D;
+  static_assert(D == 3 || D == 4, "level_group depth");
   const int nsl = (e - b + 63) >> 6;
-  const int nslD = (nsl + D - 1) / D * D;  // whole trips of the unrolled body
   const int p = b + lane;
   const WinPay W = win_pay(A, Z0);
   // The slices rotate through c[] by unrolling D steps (a register copy of a
@@ -737,17 +754,21 @@ __device__ __forceinline__ void level_group(const TileView &A, LDS &L, int Z0, i
     __builtin_amdgcn_sched_barrier(0);
   }
   lv_gather<LV, X32>(x, x32, c[0]);
-  for (int k = 0; k < nslD; k += D) {
+  int k = 0;
+  for (; k + D <= nsl; k += D) {
     const int q = p + 64 * (k + D - 1);
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-      lv_load<LV>(W, L, q + 64 * i, e, c[(i + D - 1) % D]);
-      __builtin_amdgcn_sched_barrier(0);
-      lv_gather<LV, X32>(x, x32, c[(i + 1) % D]);
-      __builtin_amdgcn_sched_barrier(0);
-      lv_consume<LV, X32>(L, c[i]);
-    }
+    level_step<LV, X32, D, 0>(W, L, q, e, x, x32, c);
+    level_step<LV, X32, D, 1>(W, L, q, e, x, x32, c);
+    level_step<LV, X32, D, 2>(W, L, q, e, x, x32, c);
+    if constexpr (D == 4) level_step<LV, X32, D, 3>(W, L, q, e, x, x32, c);
   }
+  // the last nsl - k (< D) slices
+  const int q = p + 64 * (k + D - 1);
+  const int rest = nsl - k;
+  if (rest >= 1) level_step<LV, X32, D, 0>(W, L, q, e, x, x32, c);
+  if (rest >= 2) level_step<LV, X32, D, 1>(W, L, q, e, x, x32, c);
+  if constexpr (D == 4)
+    if (rest >= 3) level_step<LV, X32, D, 2>(W, L, q, e, x, x32, c);
 }
 
 // The staged window's groups.  all64: every position (SKIP included) at

@@ -282,17 +282,16 @@ __device__ __forceinline__ void solver_infer(const SolveArgs &a, const InferCtx 
   }
 }
 
-// Dot products whose next phase reads only the wave's own rows.  With
-// resident windows a wave's SpMV epilogue writes exactly its own rows, so
-// the values-only grid sum suffices; streamed windows may hand a heavy
-// tile-row's epilogue to another wave, so they take the full barrier.
+// Dot products whose next phase reads only the wave's own rows.  A wave's
+// SpMV epilogue writes exactly its own rows (the owner finishes its heavy
+// tile-rows), so the values-only grid sum suffices: it orders no memory, and
+// a phase that reads other waves' rows takes a grid_sync first.
 template <bool RES, int NV>
 __device__ __forceinline__ void grid_sum_own(const SolveArgs &a, double (&v)[NV], unsigned &epoch,
                                              int &slot, double *lds, int *lds_i) {
-  if constexpr (RES)
-    grid_sum_values<NV>(v, a.mbox, a.gs, epoch, lds);
-  else
-    grid_sum<NV>(v, a.partials, a.gs, epoch, slot, lds, lds_i);
+  (void)slot;
+  (void)lds_i;
+  grid_sum_values<NV>(v, a.mbox, a.gs, epoch, lds);
 }
 
 // (Re-)stages a resident wave's window with the current tile levels.
@@ -358,7 +357,9 @@ __device__ __forceinline__ double restart_rows(const SolveArgs &a, int lane, boo
 // p and s are FP32 (x, the scalars and every residual recomputation stay
 // FP64; FP64 tiles multiply the FP32 r in FP64 arithmetic); after a
 // fallback to FP64 tiles the recurrence continues in FP64 vectors.
-// Vectors: FP64 r, q (w), p, s / FP32 r32, q32 (w), p32, s32.
+// Vectors: FP64 r, q (w), p, s / FP32 r32, q32 (w), p32, s32.  With resident
+// windows w never goes to memory: the update reads it from the wave's row
+// accumulators, where the SpMV left it.
 template <int MODE, bool RES>
 __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   __shared__ double lds[kMaxGridSum * kWavesPerBlock];
@@ -432,13 +433,15 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       double gd[2] = {0.0, 0.0};
       solver_spmv<MODE, mixed>(a, a.r, a.r32, a.seg_p, lowv ? 2 : force64, ctx,
                              lane, &cnt, [&](const int i, const double wv) {
+        // (a resident window keeps w in its row accumulators until the
+        // update; a streamed one stores it)
         double rv;
         if (lowv) {
           rv = a.r32[i];
-          a.q32[i] = (float)wv;
+          if constexpr (!RES) a.q32[i] = (float)wv;
         } else {
           rv = a.r[i];
-          a.q[i] = wv;
+          if constexpr (!RES) a.q[i] = wv;
         }
         gd[0] += rv * rv;
         gd[1] += wv * rv;
@@ -550,7 +553,10 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           const bool ok = i < row1;
           r_[u] = ok ? a.r32[i] : 0.0f;
           p_[u] = ok ? a.p32[i] : 0.0f;
-          q_[u] = ok ? a.q32[i] : 0.0f;
+          if constexpr (RES)
+            q_[u] = ok ? (float)ctx.L->acc[i - row0] : 0.0f;
+          else
+            q_[u] = ok ? a.q32[i] : 0.0f;
           s_[u] = ok ? a.s32[i] : 0.0f;
           x_[u] = ok ? a.x[i] : 0.0;
         }
@@ -578,7 +584,10 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           const bool ok = i < row1;
           r_[u] = ok ? a.r[i] : 0.0;
           p_[u] = ok ? a.p[i] : 0.0;
-          q_[u] = ok ? a.q[i] : 0.0;
+          if constexpr (RES)
+            q_[u] = ok ? ctx.L->acc[i - row0] : 0.0;
+          else
+            q_[u] = ok ? a.q[i] : 0.0;
           s_[u] = ok ? a.s[i] : 0.0;
           x_[u] = ok ? a.x[i] : 0.0;
         }
@@ -698,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     if (restart) {
       if (++restarts > kMaxRestart) { status = MF_ERR_BREAKDOWN; break; }
       // x of every wave complete (the sums before this point order values only)
-      if constexpr (RES) grid_sync(a.gs, epoch, &lds_i);
+      grid_sync(a.gs, epoch, &lds_i);
       double tr[1] = {restart_rows(a, lane, mixed, true, ctx)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
@@ -804,7 +813,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
                             (rr_new <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
       if (converged || periodic || omega == 0.0 || !isfinite(rr_new)) {
         double tr[2] = {0.0, 0.0};
-        if constexpr (RES) grid_sync(a.gs, epoch, &lds_i);  // x of every wave complete
+        grid_sync(a.gs, epoch, &lds_i);  // x of every wave complete
         solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, ctx, lane, nullptr,
           [&](const int i, const double ax) {
             const double rv = a.b[i] - ax;
@@ -877,7 +886,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     }
   }
   if (mixed) flush_hist(cnt, a.hist, lds, RES);
-  if constexpr (RES) grid_sync(a.gs, epoch, &lds_i);  // x of every wave complete
+  grid_sync(a.gs, epoch, &lds_i);  // x of every wave complete
   double fin[1] = {true_residual_rows(a, lane, nullptr, ctx)};
   grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1205,10 +1214,16 @@ struct Workspace {
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// Successive work vectors start this many bytes further apart than their
+// sizes: with power-of-two lengths, rows of equal index in different vectors
+// would otherwise share their address bits below the vector size, and so the
+// same HBM channels, in every streaming pass over several vectors.
+constexpr size_t kVecSkew = 4096 + 256;
+
 // Carves the handle's device workspace (grown on demand).
 mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
-  size_t need = 6 * align256(n * 8) + 4 * align256(n * 4) + 2 * align256(segs) + align256(segs * 4) +
+  size_t need = 6 * align256(n * 8) + 4 * align256(n * 4) + 10 * kVecSkew + 2 * align256(segs) + align256(segs * 4) +
                 align256(2 * kMaxGridSum * (size_t)G * 8) + align256(((size_t)G + 1) * kMboxWords * 8) +
                 align256(sizeof(GridSync)) + align256(kStCount * 8) +
                 align256(5 * 8);
@@ -1221,16 +1236,17 @@ mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   }
   char *c = (char *)t->work;
   auto take = [&](size_t bytes) { char *p = c; c += align256(bytes); return (void *)p; };
-  w.r = (double *)take(n * 8);
-  w.p = (double *)take(n * 8);
-  w.q = (double *)take(n * 8);
-  w.rh = (double *)take(n * 8);
-  w.s = (double *)take(n * 8);
-  w.tv = (double *)take(n * 8);
-  w.p32 = (float *)take(n * 4);
-  w.s32 = (float *)take(n * 4);
-  w.r32 = (float *)take(n * 4);
-  w.q32 = (float *)take(n * 4);
+  auto vec = [&](size_t bytes) { void *p = take(bytes); c += kVecSkew; return p; };
+  w.r = (double *)vec(n * 8);
+  w.p = (double *)vec(n * 8);
+  w.q = (double *)vec(n * 8);
+  w.rh = (double *)vec(n * 8);
+  w.s = (double *)vec(n * 8);
+  w.tv = (double *)vec(n * 8);
+  w.p32 = (float *)vec(n * 4);
+  w.s32 = (float *)vec(n * 4);
+  w.r32 = (float *)vec(n * 4);
+  w.q32 = (float *)vec(n * 4);
   w.seg_p = (uint8_t *)take(segs);
   w.seg_s = (uint8_t *)take(segs);
   w.seg2 = (float *)take(segs * 4);
@@ -1674,8 +1690,8 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
     MF_HIP(hipMemsetAsync(w.seg_s, MF_PREC_FP64, segs, st));
     // Padding entries of the work vectors must be zero: tiles whose columns
     // lie beyond the rows gather them.
-    MF_HIP(hipMemsetAsync(w.r, 0, 6 * align256(n * 8), st));
-    MF_HIP(hipMemsetAsync(w.p32, 0, 4 * align256(n * 4), st));
+    MF_HIP(hipMemsetAsync(w.r, 0, (size_t)((char *)w.tv - (char *)w.r) + n * 8, st));
+    MF_HIP(hipMemsetAsync(w.p32, 0, (size_t)((char *)w.q32 - (char *)w.p32) + n * 4, st));
   }
   if (opt.policy == MF_POLICY_FIXED)
     MF_HIP(hipMemcpyAsync(t->dev.prec, t->dev.prec_user, t->tilenum, hipMemcpyDeviceToDevice, st));
