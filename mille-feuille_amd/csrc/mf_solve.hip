@@ -70,6 +70,7 @@ struct SolveArgs {
   double *rh, *s, *tv;        // BiCGSTAB
   float *p32, *s32;
   float *r32, *q32;           // mixed CG: r and q in FP32 (p in p32)
+  float *rh32, *t32;          // mixed BiCGSTAB: r^ and t in FP32 (v in q32)
   uint8_t *seg_p, *seg_s;     // per-segment levels (rule policy)
   const int *wave_tr;
   const int *wave_win;       // per wave, bounds of its SpMV descriptors
@@ -326,7 +327,8 @@ __device__ __forceinline__ double true_residual_rows(const SolveArgs &a, int lan
 
 // Restart helper: r = b - A x in FP64 for the wave's SpMV share, p = r (and
 // r^ = r for BiCGSTAB).  Returns the wave's share of ||r||^2.
-// lowv: the mixed CG's FP32 vectors r32 = p32 = (float) r instead.
+// lowv: the mixed solvers' FP32 vectors r32 = p32 (= rh32) = (float) r
+// instead.
 template <bool RES>
 __device__ __forceinline__ double restart_rows(const SolveArgs &a, int lane, bool mixed, bool bicg,
                                                WaveCtx<RES> &ctx, bool lowv = false) {
@@ -338,6 +340,7 @@ __device__ __forceinline__ double restart_rows(const SolveArgs &a, int lane, boo
         const float rf = (float)rv;
         a.r32[i] = rf;
         a.p32[i] = rf;
+        if (bicg) a.rh32[i] = rf;
         rv = rf;
       } else {
         a.r[i] = rv;
@@ -643,11 +646,19 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
 }
 
 // BiCGSTAB (van der Vorst) with the reference's scalar formulas
-// (src/Mille-feuille-bicg.h:358-460).  v lives in a.q, t in a.tv.  A
-// breakdown (v.r^ = 0, omega = 0 or rho = 0) or, for the mixed policies, a
-// stagnating FP64 residual restarts the recurrence from the FP64 residual
-// with r^ = r (and every tile FP64); more than kMaxRestart restarts report a
-// breakdown.
+// (src/Mille-feuille-bicg.h:358-460).  A breakdown (v.r^ = 0, omega = 0 or
+// rho = 0) or, for the mixed policies, a stagnating FP64 residual restarts
+// the recurrence from the FP64 residual with r^ = r (and every tile FP64);
+// more than kMaxRestart restarts report a breakdown.
+//
+// With a mixed policy the work vectors r, r^, p, s, v and t are FP32 (x,
+// the scalars, the dot products and every residual recomputation stay FP64;
+// FP64 tiles multiply the FP32 vectors in FP64 arithmetic), as in the mixed
+// reference (src/main-bicg-mixed.cu:122-723); after a fallback to FP64 tiles
+// the recurrence restarts in FP64 vectors.  FP64 vectors: r, rh, p, s, v in
+// a.q, t in a.tv; FP32: r32, rh32, p32, s32, v in q32, t in t32.  With
+// resident windows t never goes to memory: the x and r update reads it from
+// the row accumulators the SpMV t = A s left it in.
 constexpr int kMaxRestart = 8;
 
 template <int MODE, bool RES>
@@ -671,15 +682,24 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   WaveCtx<RES> ctx;
   ctx.L = &rl[threadIdx.x >> 6];
   solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
+  bool lowv = mixed;  // FP32 vectors (until a fallback to FP64 tiles)
 
   double red[2] = {0.0, 0.0};
   solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, ctx, lane, nullptr,
     [&](const int i, const double ax) {
-      const double bv = a.b[i], rv = bv - ax;
-      a.r[i] = rv;
-      a.rh[i] = rv;
-      a.p[i] = rv;
-      if (mixed) a.p32[i] = (float)rv;
+      const double bv = a.b[i];
+      double rv = bv - ax;
+      if (mixed) {
+        const float rf = (float)rv;
+        a.r32[i] = rf;
+        a.rh32[i] = rf;
+        a.p32[i] = rf;
+        rv = rf;
+      } else {
+        a.r[i] = rv;
+        a.rh[i] = rv;
+        a.p[i] = rv;
+      }
       red[0] += rv * rv;
       red[1] += bv * bv;
     });
@@ -687,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     // segment norms of p = r0 for the first inference
     for (int base = row0; base < row1; base += 64) {
       const int i = base + lane;
-      const double pv = i < row1 ? a.p[i] : 0.0;
+      const double pv = i < row1 ? (double)a.p32[i] : 0.0;
       const double d2 = group16_sum(pv * pv);
       if ((lane & 15) == 0 && i < row1) a.seg2[i >> 4] = (float)d2;
     }
@@ -706,9 +726,11 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   while (!done) {
     if (restart) {
       if (++restarts > kMaxRestart) { status = MF_ERR_BREAKDOWN; break; }
+      // leaving the FP32 vectors: the FP64 recurrence restarts from here
+      if (lowv && force64) lowv = false;
       // x of every wave complete (the sums before this point order values only)
       grid_sync(a.gs, epoch, &lds_i);
-      double tr[1] = {restart_rows(a, lane, mixed, true, ctx)};
+      double tr[1] = {restart_rows(a, lane, mixed, true, ctx, lowv)};
       grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
       ++refresh;
       rr = rho = tr[0];
@@ -725,10 +747,15 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
       solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
     }
     infer = false;
-    solver_spmv<MODE, mixed>(a, a.p, a.p32, a.seg_p, force64, ctx, lane, &cnt,
+    solver_spmv<MODE, mixed>(a, a.p, a.p32, a.seg_p, lowv ? 2 : force64, ctx, lane, &cnt,
       [&](const int i, const double vv) {
-        v[i] = vv;
-        vr[0] += vv * a.rh[i];
+        if (lowv) {
+          a.q32[i] = (float)vv;
+          vr[0] += vv * (double)a.rh32[i];
+        } else {
+          v[i] = vv;
+          vr[0] += vv * a.rh[i];
+        }
       });
     grid_sum_own<RES>(a, vr, epoch, slot, lds, &lds_i);  // next: own rows of s
     if (!(vr[0] != 0.0) || !isfinite(vr[0])) {
@@ -740,27 +767,49 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     // s = r - alpha v (rule policy: |s_new - s_old| per segment)
     const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
     for (int base0 = row0; base0 < row1; base0 += 64 * kUpd) {
-      double so[kUpd], ro[kUpd], vo[kUpd];
+      double so[kUpd], sn[kUpd];
+      if (lowv) {
+        float s_[kUpd], r_[kUpd], v_[kUpd];
 #pragma unroll
-      for (int u = 0; u < kUpd; ++u) {
-        const int i = base0 + 64 * u + lane;
-        const bool ok = i < row1;
-        so[u] = ok ? a.s[i] : 0.0;
-        ro[u] = ok ? a.r[i] : 0.0;
-        vo[u] = ok ? v[i] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < kUpd; ++u) {
-        const int i = base0 + 64 * u + lane;
-        double d = 0.0;
-        if (i < row1) {
-          const double sv = ro[u] - alpha * vo[u];
-          a.s[i] = sv;
-          if (mixed) a.s32[i] = (float)sv;
-          d = fabs(sv - so[u]);
+        for (int u = 0; u < kUpd; ++u) {
+          const int i = base0 + 64 * u + lane;
+          const bool ok = i < row1;
+          s_[u] = ok && upd ? a.s32[i] : 0.0f;
+          r_[u] = ok ? a.r32[i] : 0.0f;
+          v_[u] = ok ? a.q32[i] : 0.0f;
         }
-        if (upd) {
-          d = group16_max(d);
+#pragma unroll
+        for (int u = 0; u < kUpd; ++u) {
+          const int i = base0 + 64 * u + lane;
+          const float sf = (float)((double)r_[u] - alpha * (double)v_[u]);
+          if (i < row1) a.s32[i] = sf;
+          so[u] = s_[u];
+          sn[u] = sf;
+        }
+      } else {
+        double s_[kUpd], r_[kUpd], v_[kUpd];
+#pragma unroll
+        for (int u = 0; u < kUpd; ++u) {
+          const int i = base0 + 64 * u + lane;
+          const bool ok = i < row1;
+          s_[u] = ok && upd ? a.s[i] : 0.0;
+          r_[u] = ok ? a.r[i] : 0.0;
+          v_[u] = ok ? v[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kUpd; ++u) {
+          const int i = base0 + 64 * u + lane;
+          const double sv = r_[u] - alpha * v_[u];
+          if (i < row1) a.s[i] = sv;
+          so[u] = s_[u];
+          sn[u] = sv;
+        }
+      }
+      if (upd) {
+#pragma unroll
+        for (int u = 0; u < kUpd; ++u) {
+          const int i = base0 + 64 * u + lane;
+          const double d = group16_max(i < row1 ? fabs(sn[u] - so[u]) : 0.0);
           if ((lane & 15) == 0 && i < row1) a.seg_s[i >> 4] = (uint8_t)rule_level(d);
         }
       }
@@ -768,10 +817,17 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     grid_sync(a.gs, epoch, &lds_i);
     // t = A s, t.s and t.t
     double tw[2] = {0.0, 0.0};
-    solver_spmv<MODE, mixed>(a, a.s, a.s32, a.seg_s, force64, ctx, lane, &cnt,
+    solver_spmv<MODE, mixed>(a, a.s, a.s32, a.seg_s, lowv ? 2 : force64, ctx, lane, &cnt,
       [&](const int i, const double tvv) {
-        t[i] = tvv;
-        tw[0] += tvv * a.s[i];
+        double sv;
+        if (lowv) {
+          if constexpr (!RES) a.t32[i] = (float)tvv;
+          sv = a.s32[i];
+        } else {
+          if constexpr (!RES) t[i] = tvv;
+          sv = a.s[i];
+        }
+        tw[0] += tvv * sv;
         tw[1] += tvv * tvv;
       });
     grid_sum_own<RES>(a, tw, epoch, slot, lds, &lds_i);  // next: own rows of x and r
@@ -786,18 +842,33 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
         const int i = base + 64 * u;
         const bool ok = i < row1;
         x_[u] = ok ? a.x[i] : 0.0;
-        p_[u] = ok ? a.p[i] : 0.0;
-        s_[u] = ok ? a.s[i] : 0.0;
-        t_[u] = ok ? t[i] : 0.0;
-        h_[u] = ok ? a.rh[i] : 0.0;
+        if (lowv) {
+          p_[u] = ok ? (double)a.p32[i] : 0.0;
+          s_[u] = ok ? (double)a.s32[i] : 0.0;
+          h_[u] = ok ? (double)a.rh32[i] : 0.0;
+        } else {
+          p_[u] = ok ? a.p[i] : 0.0;
+          s_[u] = ok ? a.s[i] : 0.0;
+          h_[u] = ok ? a.rh[i] : 0.0;
+        }
+        if constexpr (RES)
+          t_[u] = ok ? ctx.L->acc[i - row0] : 0.0;
+        else
+          t_[u] = ok ? (lowv ? (double)a.t32[i] : t[i]) : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < kUpd; ++u) {
         const int i = base + 64 * u;
         if (i < row1) {
           a.x[i] = x_[u] + (alpha * p_[u] + omega * s_[u]);
-          const double rv = s_[u] - omega * t_[u];
-          a.r[i] = rv;
+          double rv = s_[u] - omega * t_[u];
+          if (lowv) {
+            const float rf = (float)rv;
+            a.r32[i] = rf;
+            rv = rf;
+          } else {
+            a.r[i] = rv;
+          }
           nw[0] += rv * h_[u];
           nw[1] += rv * rv;
         }
@@ -816,10 +887,19 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
         grid_sync(a.gs, epoch, &lds_i);  // x of every wave complete
         solver_spmv<kAll64, false>(a, a.x, nullptr, nullptr, 0, ctx, lane, nullptr,
           [&](const int i, const double ax) {
-            const double rv = a.b[i] - ax;
-            a.r[i] = rv;
+            double rv = a.b[i] - ax;
+            double hv;
+            if (lowv) {
+              const float rf = (float)rv;
+              a.r32[i] = rf;
+              rv = rf;
+              hv = a.rh32[i];
+            } else {
+              a.r[i] = rv;
+              hv = a.rh[i];
+            }
             tr[0] += rv * rv;
-            tr[1] += rv * a.rh[i];
+            tr[1] += rv * hv;
           });
         grid_sum<2>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
         ++refresh;
@@ -847,25 +927,49 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     const bool lupd = learned && !force64 && (it % a.check_interval) == 0;
     double pp[1] = {0.0};
     for (int base0 = row0; base0 < row1; base0 += 64 * kUpd) {
-      double po[kUpd], ro[kUpd], vo[kUpd];
+      double po[kUpd], pn[kUpd];
+      if (lowv) {
+        float p_[kUpd], r_[kUpd], v_[kUpd];
 #pragma unroll
-      for (int u = 0; u < kUpd; ++u) {
-        const int i = base0 + 64 * u + lane;
-        const bool ok = i < row1;
-        po[u] = ok ? a.p[i] : 0.0;
-        ro[u] = ok ? a.r[i] : 0.0;
-        vo[u] = ok ? v[i] : 0.0;
+        for (int u = 0; u < kUpd; ++u) {
+          const int i = base0 + 64 * u + lane;
+          const bool ok = i < row1;
+          p_[u] = ok ? a.p32[i] : 0.0f;
+          r_[u] = ok ? a.r32[i] : 0.0f;
+          v_[u] = ok ? a.q32[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kUpd; ++u) {
+          const int i = base0 + 64 * u + lane;
+          const float pf =
+              (float)((double)r_[u] + beta * ((double)p_[u] - omega * (double)v_[u]));
+          if (i < row1) a.p32[i] = pf;
+          po[u] = p_[u];
+          pn[u] = pf;
+        }
+      } else {
+        double p_[kUpd], r_[kUpd], v_[kUpd];
+#pragma unroll
+        for (int u = 0; u < kUpd; ++u) {
+          const int i = base0 + 64 * u + lane;
+          const bool ok = i < row1;
+          p_[u] = ok ? a.p[i] : 0.0;
+          r_[u] = ok ? a.r[i] : 0.0;
+          v_[u] = ok ? v[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kUpd; ++u) {
+          const int i = base0 + 64 * u + lane;
+          const double pv = r_[u] + beta * (p_[u] - omega * v_[u]);
+          if (i < row1) a.p[i] = pv;
+          po[u] = p_[u];
+          pn[u] = pv;
+        }
       }
 #pragma unroll
       for (int u = 0; u < kUpd; ++u) {
         const int i = base0 + 64 * u + lane;
-        double d = 0.0;
-        if (i < row1) {
-          const double pv = ro[u] + beta * (po[u] - omega * vo[u]);
-          a.p[i] = pv;
-          if (mixed) a.p32[i] = (float)pv;
-          d = lupd ? pv * pv : fabs(pv - po[u]);
-        }
+        double d = i < row1 ? (lupd ? pn[u] * pn[u] : fabs(pn[u] - po[u])) : 0.0;
         if (updp) {
           d = group16_max(d);
           if ((lane & 15) == 0 && i < row1) a.seg_p[i >> 4] = (uint8_t)rule_level(d);
@@ -1202,7 +1306,7 @@ namespace {
 
 struct Workspace {
   double *r, *p, *q, *rh, *s, *tv;
-  float *p32, *s32, *r32, *q32;
+  float *p32, *s32, *r32, *q32, *rh32 = nullptr, *t32 = nullptr;
   uint8_t *seg_p, *seg_s;
   float *seg2;
   double *partials;
@@ -1223,7 +1327,7 @@ constexpr size_t kVecSkew = 4096 + 256;
 // Carves the handle's device workspace (grown on demand).
 mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   const size_t n = t->dev.npad, segs = t->tilen > t->tilem ? t->tilen : t->tilem;
-  size_t need = 6 * align256(n * 8) + 4 * align256(n * 4) + 10 * kVecSkew + 2 * align256(segs) + align256(segs * 4) +
+  size_t need = 6 * align256(n * 8) + 6 * align256(n * 4) + 12 * kVecSkew + 2 * align256(segs) + align256(segs * 4) +
                 align256(2 * kMaxGridSum * (size_t)G * 8) + align256(((size_t)G + 1) * kMboxWords * 8) +
                 align256(sizeof(GridSync)) + align256(kStCount * 8) +
                 align256(5 * 8);
@@ -1247,6 +1351,8 @@ mf_status get_workspace(mf_tiled *t, int G, Workspace &w) {
   w.s32 = (float *)vec(n * 4);
   w.r32 = (float *)vec(n * 4);
   w.q32 = (float *)vec(n * 4);
+  w.rh32 = (float *)vec(n * 4);
+  w.t32 = (float *)vec(n * 4);
   w.seg_p = (uint8_t *)take(segs);
   w.seg_s = (uint8_t *)take(segs);
   w.seg2 = (float *)take(segs * 4);
@@ -1691,7 +1797,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
     // Padding entries of the work vectors must be zero: tiles whose columns
     // lie beyond the rows gather them.
     MF_HIP(hipMemsetAsync(w.r, 0, (size_t)((char *)w.tv - (char *)w.r) + n * 8, st));
-    MF_HIP(hipMemsetAsync(w.p32, 0, (size_t)((char *)w.q32 - (char *)w.p32) + n * 4, st));
+    MF_HIP(hipMemsetAsync(w.p32, 0, (size_t)((char *)w.t32 - (char *)w.p32) + n * 4, st));
   }
   if (opt.policy == MF_POLICY_FIXED)
     MF_HIP(hipMemcpyAsync(t->dev.prec, t->dev.prec_user, t->tilenum, hipMemcpyDeviceToDevice, st));
@@ -1706,7 +1812,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   a.b = d_b;
   a.x = d_x;
   a.r = w.r; a.p = w.p; a.q = w.q; a.rh = w.rh; a.s = w.s; a.tv = w.tv;
-  a.p32 = w.p32; a.s32 = w.s32; a.r32 = w.r32; a.q32 = w.q32;
+  a.p32 = w.p32; a.s32 = w.s32; a.r32 = w.r32; a.q32 = w.q32; a.rh32 = w.rh32; a.t32 = w.t32;
   a.seg_p = w.seg_p; a.seg_s = w.seg_s;
   a.wave_tr = sch->wave_tr;
   a.wave_win = sch->wave_win;
