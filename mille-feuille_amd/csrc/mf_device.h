@@ -110,6 +110,7 @@ struct WaveLds {
   uint8_t trow[kWinTiles];  // tile-row relative to the window's first tile-row
   uint8_t tprec[kWinTiles];
   uint8_t ctile[kWinChunks];  // window-relative tile of each 4-entry chunk
+  static constexpr int kAccStride = 16;  // accumulators per tile-row
   double acc[kWinRows * 16];
 };
 
@@ -614,7 +615,7 @@ __device__ __forceinline__ void lv_load(const WinPay &W, const LDS &L, int pos, 
   const int c = p + (te >> 13) - 1024;
   s.valid = valid;
   s.tcol = L.tcol[j];
-  s.ro = L.trow[j] * 16;
+  s.ro = L.trow[j] * LDS::kAccStride;
   s.sc = (LV == MF_PREC_FP16 || LV == MF_PREC_INT8) ? L.tscale[j] : 1.0f;
   const uint32_t c4 = 4u * (uint32_t)c;  // first slot of the chunk in the window
   s.rc4 = ld_at<uint32_t>(W.rc, c4);
@@ -815,8 +816,15 @@ struct ResLds {
   int tend[kResTiles];        // entry end | position offset (level groups)
   uint8_t trow[kResTiles];    // tile-row relative to the wave's first
   uint8_t ctile[kResChunks];  // tile of each group position
-  double acc[kResRows * 16];
+  // Row sums, 17 per tile-row: rows of equal index in different tile-rows
+  // (which the lanes of one slice often add to at once) fall in different
+  // LDS banks.
+  static constexpr int kAccStride = 17;
+  double acc[kResRows * kAccStride];
 };
+
+// Accumulator of window row i (relative to the window's first row).
+__device__ __forceinline__ int res_acc_index(int i) { return (i >> 4) * ResLds::kAccStride + (i & 15); }
 
 // The wave's resident window: tile-rows [tr0, tr1), tiles [ta, ta + nt),
 // payload from slot Z0, its level groups and per-level tile counts (for the
@@ -903,7 +911,7 @@ __device__ __forceinline__ void res_spmv(const TileView &A, ResLds &L, const Res
                                          const float *__restrict__ x32, bool all64, bool x32only,
                                          int lane, PrecCount *cnt, Epi epi) {
   const int nr = (w.tr1 - w.tr0) * 16;
-  for (int i = lane; i < nr; i += 64) L.acc[i] = 0.0;
+  for (int i = lane; i < (w.tr1 - w.tr0) * ResLds::kAccStride; i += 64) L.acc[i] = 0.0;
   // (the counts are the wave's, alike in every lane: kept in scalar
   // registers, and the histogram flush takes lane 0's)
   if (COUNT && all64) {
@@ -918,7 +926,7 @@ __device__ __forceinline__ void res_spmv(const TileView &A, ResLds &L, const Res
   __builtin_amdgcn_wave_barrier();
   level_groups(A, L, w.Z0, w.g, x, x32, all64, x32only, lane);
   __builtin_amdgcn_wave_barrier();
-  for (int i = lane; i < nr; i += 64) epi(16 * w.tr0 + i, L.acc[i]);
+  for (int i = lane; i < nr; i += 64) epi(16 * w.tr0 + i, L.acc[res_acc_index(i)]);
   __builtin_amdgcn_wave_barrier();
 }
 

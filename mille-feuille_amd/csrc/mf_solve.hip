@@ -557,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           r_[u] = ok ? a.r32[i] : 0.0f;
           p_[u] = ok ? a.p32[i] : 0.0f;
           if constexpr (RES)
-            q_[u] = ok ? (float)ctx.L->acc[i - row0] : 0.0f;
+            q_[u] = ok ? (float)ctx.L->acc[res_acc_index(i - row0)] : 0.0f;
           else
             q_[u] = ok ? a.q32[i] : 0.0f;
           s_[u] = ok ? a.s32[i] : 0.0f;
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           r_[u] = ok ? a.r[i] : 0.0;
           p_[u] = ok ? a.p[i] : 0.0;
           if constexpr (RES)
-            q_[u] = ok ? ctx.L->acc[i - row0] : 0.0;
+            q_[u] = ok ? ctx.L->acc[res_acc_index(i - row0)] : 0.0;
           else
             q_[u] = ok ? a.q[i] : 0.0;
           s_[u] = ok ? a.s[i] : 0.0;
@@ -852,7 +852,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
           h_[u] = ok ? a.rh[i] : 0.0;
         }
         if constexpr (RES)
-          t_[u] = ok ? ctx.L->acc[i - row0] : 0.0;
+          t_[u] = ok ? ctx.L->acc[res_acc_index(i - row0)] : 0.0;
         else
           t_[u] = ok ? (lowv ? (double)a.t32[i] : t[i]) : 0.0;
       }
