@@ -227,10 +227,14 @@ typedef struct mf_solve_opts {
   int blocks;           /* persistent grid size override (0 = occupancy) */
   int fixed_iters;      /* >0: run exactly this many iterations, no test
                            (reference timing mode, src/Mille-feuille-cg.h:426) */
-  int form;             /* CG kernel form: MF_FORM_AUTO, MF_FORM_MEMORY (vectors
-                           in HBM, two syncs per iteration) or MF_FORM_RESIDENT
-                           (vectors in registers/LDS, two syncs; needs regular
-                           tile-rows and at most 8192 rows per CU) */
+  int form;             /* solver SpMV form: MF_FORM_MEMORY (every SpMV reads
+                           the tile metadata of each wave's windows from HBM),
+                           MF_FORM_RESIDENT or MF_FORM_AUTO (each wave keeps
+                           its share's tile metadata in LDS for the whole
+                           solve when every share fits a window of 40
+                           tile-rows, 256 tiles and 1024 chunks with no heavy
+                           tile-row; else as MEMORY).  The report's form_used
+                           says which ran. */
   int batch_waves;      /* mf_batch_cg_solve: waves per (system, right-hand
                            side) item, 4, 8 or 16 (0 -> 8; mf_tune_batch
                            picks one) */
@@ -276,7 +280,7 @@ mf_status mf_cg_solve(mf_tiled *t, const double *d_b, double *d_x,
  * rejected with MF_ERR_ARG.  The caller passes d_b and d_x as for
  * mf_cg_solve.  Every call on a device shares that array, so calls that
  * overlap on one device race; the report's time_ms is 0 (no events are
- * created); the resident register form (MF_FORM_RESIDENT) is not used. */
+ * created). */
 mf_status mf_cg_solve_noalloc(mf_tiled *t, const double *d_b, double *d_x,
                               const mf_solve_opts *opts, mf_report *rep);
 

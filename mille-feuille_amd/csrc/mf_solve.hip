@@ -1004,273 +1004,6 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// Register-resident CG (Chronopoulos-Gear form).
-//
-// Same mathematics as textbook CG with one SpMV (w = A r) and one fused pair
-// of dot products (gamma = r.r, delta = w.r) per iteration, so an iteration
-// needs two grid syncs instead of three:
-//   beta = gamma/gamma_old, alpha = gamma/(delta - beta*gamma/alpha_old),
-//   p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s   (s = A p).
-// Every wave owns 16*(tr1-tr0) <= 64*K consecutive rows and keeps p, s and
-// w = A r of those rows in registers (lane L holds rows L, L+64, ...) and x
-// in wave-private LDS.  The SpMV is the row-group form, whose pass k leaves
-// row 64k + L of the wave's block in lane L: the products land directly in
-// w[k].  The only vector stored per iteration is r (plus its FP32 copy for
-// low-precision tiles), which the next SpMV gathers; the wave reads its own
-// rows of r back (L2-resident: the SpMV has just gathered them).  FP64 residual recomputation replaces r by b - A x and
-// s by A p, both in FP64, which keeps the recurrences consistent without
-// restarting (residual replacement); a stagnating mixed run falls back to
-// FP64 tiles and restarts from the recomputed residual.
-// ---------------------------------------------------------------------------
-// Two shapes, one block per CU: 16 waves with 8 rows per lane (up to 128
-// VGPRs; 8192 rows per CU) or 8 waves with 16 rows per lane (up to 256
-// VGPRs; 8192 rows per CU as well, for kernels that need the registers).
-constexpr int kResBlockA = 1024, kResKA = 8;
-constexpr int kResBlockB = 512, kResKB = 16;
-
-// A copy of v the compiler cannot see through: per-row addresses derived from
-// it are recomputed where they are used instead of being hoisted out of the
-// solver loop (sixteen hoisted 64-bit addresses per vector would not fit in
-// the register budget of a resident wave).
-__device__ __forceinline__ int opaque(int v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
-
-template <int MODE, int K, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_cg_res(SolveArgs a) {
-  constexpr int NW = BLOCK / 64;
-  __shared__ double lds[kMaxGridSum * NW];
-  __shared__ int lds_i;
-  __shared__ double xbuf[NW * K * 64];
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int wid = blockIdx.x * NW + wv;
-  const int tr0 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid]);
-  const int tr1 = __builtin_amdgcn_readfirstlane(a.wave_tr[wid + 1]);
-  const int row0 = tr0 * 16;
-  const int nrows = (tr1 - tr0) * 16;
-  double *X = xbuf + wv * K * 64;  // x of the wave's rows
-  constexpr bool mixed = MODE != kAll64;
-  const bool learned = MODE == kPerTile && a.mlp != nullptr;
-  unsigned epoch = 0;
-  int slot = 0;
-  PrecCount cnt;
-  double p[K], s[K], w[K];
-
-  // r = b - A x0
-  rowgroup_passes<kAll64, false, K>(a.A, tr0, tr1, lane, a.x, nullptr, nullptr, 0, nullptr, w);
-  double red[2] = {0.0, 0.0};
-  const int ln0 = opaque(lane);
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int li = ln0 + 64 * k;
-    double d2 = 0.0;
-    p[k] = s[k] = 0.0;
-    if (li < nrows) {
-      const int i = row0 + li;
-      const double bv = a.b[i];
-      X[li] = a.x[i];
-      const double rv = bv - w[k];
-      red[0] += rv * rv;
-      red[1] += bv * bv;
-      a.r[i] = rv;
-      if (mixed) a.p32[i] = (float)rv;
-      d2 = rv * rv;
-    }
-    if (learned) {
-      d2 = group16_sum(d2);
-      if ((lane & 15) == 0 && li < nrows) a.seg2[(row0 + li) >> 4] = (float)d2;
-    }
-  }
-  grid_sum<2>(red, a.partials, a.gs, epoch, slot, lds, &lds_i);
-  double rr = red[0];
-  const double bb = red[1];
-  const double stop2 = uniform_double(a.tol2 * bb);
-  double rr_ref = rr, gamma_old = 1.0, alpha_old = 1.0;
-  bool first = true;
-  int it = 0, it_ref = 0, refresh = 0, force64 = 0, status = 0;
-  bool done = bb == 0.0 || (a.fixed_iters <= 0 && rr <= stop2);
-
-  bool pending = false;  // recompute the FP64 residual before the next step
-  bool infer = learned;  // learned policy: new levels in the next SpMV
-  while (!done) {
-    double gamma = 0.0, delta = 0.0;
-    if (!pending) {
-      // w = A r (r published by every wave before the last sync); the
-      // learned policy re-evaluates the tile levels every check interval
-      // from the segment norms of r written before that sync.
-      if (infer && !force64) wave_infer(a.A, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
-      infer = false;
-      rowgroup_passes<MODE, mixed, K>(a.A, tr0, tr1, lane, a.r, a.p32, a.seg_p, force64, &cnt, w);
-      double gd[2] = {0.0, 0.0};
-      const int ln1 = opaque(lane);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int li = ln1 + 64 * k;
-        if (li < nrows) {
-          const double rv = a.r[row0 + li];
-          gd[0] += rv * rv;
-          gd[1] += w[k] * rv;
-        }
-      }
-      grid_sum<2>(gd, a.partials, a.gs, epoch, slot, lds, &lds_i);
-      gamma = gd[0];
-      delta = gd[1];
-      rr = gamma;
-      if (a.fixed_iters <= 0) {
-        const bool converged = gamma <= stop2;
-        const bool periodic = mixed && !force64 &&
-                              (gamma <= a.ratio2 * rr_ref || it - it_ref >= a.refresh_every);
-        pending = converged || periodic;
-        if (!pending && it >= a.maxiter) { status = MF_NOT_CONVERGED; break; }
-      }
-    }
-    if (pending) {
-      pending = false;
-      // FP64 residual replacement: publish x, r = b - A x.
-      const int ln2 = opaque(lane);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int li = ln2 + 64 * k;
-        if (li < nrows) a.x[row0 + li] = X[li];
-      }
-      grid_sync(a.gs, epoch, &lds_i);
-      rowgroup_passes<kAll64, false, K>(a.A, tr0, tr1, lane, a.x, nullptr, nullptr, 0, nullptr, w);
-      double tr[1] = {0.0};
-      const int ln3 = opaque(lane);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int li = ln3 + 64 * k;
-        if (li < nrows) {
-          const int i = row0 + li;
-          const double rv = a.b[i] - w[k];
-          tr[0] += rv * rv;
-          a.r[i] = rv;
-          if (mixed) a.p32[i] = (float)rv;
-        }
-      }
-      grid_sum<1>(tr, a.partials, a.gs, epoch, slot, lds, &lds_i);
-      ++refresh;
-      it_ref = it;
-      // A recursive residual far below the true one (a low-precision operator
-      // or rounding drifted it) leaves beta = gamma/gamma_old meaningless:
-      // restart the search direction from the recomputed residual.
-      if (tr[0] > 4.0 * rr) first = true;
-      rr = tr[0];
-      if (tr[0] <= stop2) break;
-      if (it >= a.maxiter) { status = MF_NOT_CONVERGED; break; }
-      if (mixed && !force64 && (tr[0] > 0.25 * rr_ref || refresh > a.max_refresh)) {
-        force64 = 1;
-        first = true;  // stagnation: FP64 tiles from here, restart the recurrence
-      }
-      rr_ref = tr[0];
-      if (!first) {
-        // s = A p in FP64 so that s and the new r stay consistent
-        const int ln4 = opaque(lane);
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const int li = ln4 + 64 * k;
-          if (li < nrows) a.q[row0 + li] = p[k];
-        }
-        grid_sync(a.gs, epoch, &lds_i);
-        rowgroup_passes<kAll64, false, K>(a.A, tr0, tr1, lane, a.q, nullptr, nullptr, 0, nullptr, w);
-#pragma unroll
-        for (int k = 0; k < K; ++k) s[k] = w[k];
-      }
-      continue;
-    }
-    double beta, alpha;
-    if (first) {
-      beta = 0.0;
-      alpha = gamma / delta;
-    } else {
-      beta = gamma / gamma_old;
-      alpha = gamma / (delta - beta * gamma / alpha_old);
-    }
-    if (!isfinite(alpha) || alpha == 0.0 || (mixed && !force64 && !(alpha > 0.0))) {
-      if (mixed && !force64) {
-        // The low-precision operator lost definiteness and r has been
-        // updated with it: FP64 tiles, recompute r = b - A x, restart.
-        force64 = 1;
-        first = true;
-        pending = true;
-        continue;
-      }
-      status = MF_ERR_BREAKDOWN;
-      break;
-    }
-    first = false;
-    alpha = uniform_double(alpha);
-    beta = uniform_double(beta);
-    gamma_old = gamma;
-    alpha_old = alpha;
-    ++it;
-    const bool upd = MODE == kPerSeg && (it % a.check_interval) == 0;
-    const bool lupd = learned && !force64 && (it % a.check_interval) == 0;
-    const int ln6 = opaque(lane);
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int li = ln6 + 64 * k;
-      double d = 0.0;
-      if (li < nrows) {
-        const int i = row0 + li;
-        const double rv = a.r[i];
-        p[k] = rv + beta * p[k];
-        s[k] = w[k] + beta * s[k];
-        X[li] += alpha * p[k];
-        const double rn = rv - alpha * s[k];
-        d = fabs(rn - rv);
-        a.r[i] = rn;
-        if (mixed) a.p32[i] = (float)rn;
-        if (lupd) d = rn * rn;
-      }
-      if (upd) {
-        d = group16_max(d);
-        if ((lane & 15) == 0 && li < nrows) a.seg_p[(row0 + li) >> 4] = (uint8_t)rule_level(d);
-      }
-      if (lupd) {
-        d = group16_sum(d);
-        if ((lane & 15) == 0 && li < nrows) a.seg2[(row0 + li) >> 4] = (float)d;
-      }
-    }
-    infer = lupd;
-    grid_sync(a.gs, epoch, &lds_i);
-    if (a.fixed_iters > 0 && it >= a.fixed_iters) break;
-  }
-  if (mixed) flush_hist(cnt, a.hist, lds);
-  // publish x, final FP64 residual for the report
-  const int ln7 = opaque(lane);
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int li = ln7 + 64 * k;
-    if (li < nrows) a.x[row0 + li] = X[li];
-  }
-  grid_sync(a.gs, epoch, &lds_i);
-  double fin[1] = {0.0};
-  rowgroup_passes<kAll64, false, K>(a.A, tr0, tr1, lane, a.x, nullptr, nullptr, 0, nullptr, w);
-  const int ln8 = opaque(lane);
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int li = ln8 + 64 * k;
-    if (li < nrows) {
-      const double d = a.b[row0 + li] - w[k];
-      fin[0] += d * d;
-    }
-  }
-  grid_sum<1>(fin, a.partials, a.gs, epoch, slot, lds, &lds_i);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.stats[kStIters] = it;
-    a.stats[kStRefresh] = refresh;
-    a.stats[kStFallback] = force64;
-    a.stats[kStRecursive] = rr;
-    a.stats[kStTrue] = fin[0];
-    a.stats[kStBB] = bb;
-    a.stats[kStStatus] = status;
-  }
-}
-
 // Error-bound policy: every tile takes the cheapest payload whose largest
 // quantisation error is within `theta` of the tile's largest magnitude
 // (INT8, then FP16, then FP32).  With theta at FP32 rounding (2^-24) a tile
@@ -1378,13 +1111,11 @@ mf_status grid_for(K kernel, int override_blocks, int *G, int block = kBlock, bo
   return MF_OK;
 }
 
-// Per-wave tile-row ranges: nnz-balanced (memory-resident kernels) or an
-// even split of tile-rows (register-resident CG, whose per-wave state is
-// sized by rows), and each wave's SpMV descriptors.  A tile-row larger than
-// one window and than half a wave's share of the SpMV work is heavy: it is
-// cut into pieces spread over the waves with the least work, so hub rows of
-// power-law matrices do not serialise on one wave.  Cached per (wave count,
-// kind).
+// Per-wave tile-row ranges, balanced on each tile-row's SpMV and vector-update
+// work, and each wave's SpMV descriptors.  A tile-row larger than one window
+// and than half a wave's share of the SpMV work is heavy: it is cut into
+// pieces spread over the waves with the least work, so hub rows of power-law
+// matrices do not serialise on one wave.  Cached per wave count.
 // Host side of a solver schedule: per-wave tile-row bounds, resident flags,
 // the waves' SpMV descriptors and the heavy tile-row pieces.
 struct HostSchedule {
@@ -1393,7 +1124,7 @@ struct HostSchedule {
   int npieces = 0, all_resident = 1;
 };
 
-void build_schedule(const mf_tiled *t, int nwaves, bool even_rows, HostSchedule &hsch) {
+void build_schedule(const mf_tiled *t, int nwaves, HostSchedule &hsch) {
   const int tm = t->tilem;
   auto slots = [&](int r) { return t->pay_off[t->tile_ptr[r + 1]] - t->pay_off[t->tile_ptr[r]]; };
   auto tiles = [&](int r) { return t->tile_ptr[r + 1] - t->tile_ptr[r]; };
@@ -1402,11 +1133,7 @@ void build_schedule(const mf_tiled *t, int nwaves, bool even_rows, HostSchedule 
   auto cost = [&](int r) { return (double)slots(r) + 8.0 * tiles(r) + 128.0; };
   std::vector<char> heavy(tm, 0);
   std::vector<int> b;
-  if (even_rows) {
-    const int q = (tm + nwaves - 1) / nwaves;
-    b.resize(nwaves + 1);
-    for (int w = 0; w <= nwaves; ++w) b[w] = std::min(tm, w * q);
-  } else {
+  {
     double total = 0.0;
     for (int r = 0; r < tm; ++r) total += cost(r);
     const double share = total / nwaves;
@@ -1505,8 +1232,8 @@ void build_schedule(const mf_tiled *t, int nwaves, bool even_rows, HostSchedule 
   hsch.npieces = npieces;
 }
 
-mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
-  const int key = 2 * nwaves + (even_rows ? 1 : 0);
+mf_status ensure_schedule(mf_tiled *t, int nwaves) {
+  const int key = nwaves;
   if (t->sched.wave_tr && t->sched_waves_for == key) return MF_OK;
   for (void *p : {(void *)t->sched.wave_tr, (void *)t->sched.wave_res, (void *)t->sched.wave_win, (void *)t->sched.win_desc,
                   (void *)t->sched.heavy_part, (void *)t->sched.heavy_cnt, (void *)t->sched.heavy_n,
@@ -1515,7 +1242,7 @@ mf_status ensure_schedule(mf_tiled *t, int nwaves, bool even_rows = false) {
   t->sched = Schedule();
   t->sched_waves_for = -1;
   HostSchedule hsch;
-  build_schedule(t, nwaves, even_rows, hsch);
+  build_schedule(t, nwaves, hsch);
   const std::vector<int> &b = hsch.b, &res = hsch.res, &ww = hsch.ww, &hn = hsch.hn, &hbase = hsch.hbase;
   const std::vector<WaveDesc> &wd = hsch.wd;
   const int npieces = hsch.npieces;
@@ -1698,51 +1425,15 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   const void *kern;
   int G = 0, block = kBlock;
   mf_status sst = MF_OK;
-  bool resident = false, resident_windows = false;
+  bool resident_windows = false;
   HostSchedule nsched;  // allocation-free solves: the schedule built on the host
   if (opt.form < MF_FORM_AUTO || opt.form > MF_FORM_RESIDENT) return fail(MF_ERR_ARG, "solve: unknown form");
-  // Auto picks the memory form: on MI355X (3D 7-point 128^3) it runs FP64 CG
-  // as fast as the resident form (106 vs 108 us per iteration) and mixed CG
-  // faster (152 vs 213): the resident form's 8-16 waves per CU leave the SpMV
-  // latency-bound.
-  if (!bicg && !noalloc && opt.form == MF_FORM_RESIDENT && regular_tilerows(t)) {
-    // Resident CG when every wave's even share of rows fits its registers:
-    // 16 waves x 512 rows per CU unless that kernel spills, else 8 waves x
-    // 1024 rows.
-    const void *ka = mode == kAll64 ? (const void *)k_cg_res<kAll64, kResKA, kResBlockA>
-                   : mode == kPerSeg ? (const void *)k_cg_res<kPerSeg, kResKA, kResBlockA>
-                                     : (const void *)k_cg_res<kPerTile, kResKA, kResBlockA>;
-    const void *kb = mode == kAll64 ? (const void *)k_cg_res<kAll64, kResKB, kResBlockB>
-                   : mode == kPerSeg ? (const void *)k_cg_res<kPerSeg, kResKB, kResBlockB>
-                                     : (const void *)k_cg_res<kPerTile, kResKB, kResBlockB>;
-    const struct { const void *k; int block, K; } shapes[2] = {{ka, kResBlockA, kResKA},
-                                                                {kb, kResBlockB, kResKB}};
-    for (const auto &sh : shapes) {
-      // a shape whose kernel spills registers to scratch is passed over
-      hipFuncAttributes fa;
-      MF_HIP(hipFuncGetAttributes(&fa, sh.k));
-      if (fa.localSizeBytes > 0 && &sh != &shapes[1]) continue;
-      int Gs = 0;
-      sst = grid_for(sh.k, opt.blocks, &Gs, sh.block, true);
-      if (sst != MF_OK) return sst;
-      const int nw = Gs * (sh.block / 64);
-      const int q = (t->tilem + nw - 1) / nw;
-      if (q <= 4 * sh.K) {  // 16 q rows <= 64 K
-        resident = true;
-        kern = sh.k;
-        G = Gs;
-        block = sh.block;
-        sst = ensure_schedule(t, nw, true);
-        if (sst != MF_OK) return sst;
-        break;
-      }
-    }
-  }
-  if (!resident) {
+  {
     // The grid is the occupancy of the streamed kernel (one block of 16
-    // waves per CU for both forms); the resident kernel runs when every
-    // wave's share fits a resident window (not for the rule policy's
-    // BiCGSTAB, which multiplies p and s at different segment levels).
+    // waves per CU for both forms).  Unless the memory form is asked for,
+    // the resident-window kernel runs when every wave's share fits a
+    // resident window (not for the rule policy's BiCGSTAB, which multiplies
+    // p and s at different segment levels).
     const void *kstream = bicg ? (mode == kAll64 ? (const void *)k_bicgstab<kAll64, false>
                                   : mode == kPerSeg ? (const void *)k_bicgstab<kPerSeg, false>
                                                     : (const void *)k_bicgstab<kPerTile, false>)
@@ -1752,14 +1443,14 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
     sst = grid_for(kstream, opt.blocks, &G);
     if (sst != MF_OK) return sst;
     if (noalloc) {
-      build_schedule(t, G * kWavesPerBlock, false, nsched);
+      build_schedule(t, G * kWavesPerBlock, nsched);
     } else {
       sst = ensure_schedule(t, G * kWavesPerBlock);
       if (sst != MF_OK) return sst;
     }
     kern = kstream;
     const int all_res = noalloc ? nsched.all_resident : t->sched.all_resident;
-    if (all_res && !(bicg && mode == kPerSeg) && !std::getenv("MF_NO_RESIDENT_WINDOWS")) {
+    if (all_res && opt.form != MF_FORM_MEMORY && !(bicg && mode == kPerSeg)) {
       int Gr = 0;
       const void *kres = bicg ? (mode == kAll64 ? (const void *)k_bicgstab<kAll64, true>
                                                 : (const void *)k_bicgstab<kPerTile, true>)
@@ -1918,7 +1609,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   r.time_ms = ms;
   for (int i = 0; i < 5; ++i) r.prec_hist[i] = (long long)hist[i];
   r.grid_blocks = G;
-  r.form_used = resident ? MF_FORM_RESIDENT : MF_FORM_MEMORY;
+  r.form_used = resident_windows ? MF_FORM_RESIDENT : MF_FORM_MEMORY;
   r.policy_used = opt.policy;
   int ret = status;
   if (ret == MF_OK && opt.fixed_iters <= 0 && !(r.rel_residual <= opt.tol)) ret = MF_NOT_CONVERGED;
