@@ -1420,25 +1420,32 @@ __device__ __forceinline__ void rowgroup_passes(const TileView &A, int tr0, int 
 // batch and the levels go to ic.prec_out, where the next per-tile SpMV pass
 // of the same wave reads them (the closing workgroup fence orders the stores
 // before those loads).  Callers infer exactly the tiles their own SpMV reads,
-// so no level is written while another wave may read it.
-__device__ __forceinline__ void wave_infer_tiles(const TileView &A, const InferCtx &ic, int t0, int t1,
+// so no level is written while another wave may read it.  Returns (alike in
+// every lane) whether any of the tiles changed level.
+__device__ __forceinline__ bool wave_infer_tiles(const TileView &A, const InferCtx &ic, int t0, int t1,
                                                  int lane) {
+  bool changed = false;
   for (int g = t0; g < t1; g += 16) {
     const int t = g + (lane & 15);
     const bool valid = t < t1;
     const int tt = valid ? t : t0;
     const half4_t f = infer_features(ic, tt, (lane >= 16 && valid) ? A.tile_col[tt] : 0, lane, valid);
     const float4_t z = mlp_eval16(ic.w, lane, f);
-    if (lane < 16 && valid) ic.prec_out[t] = (uint8_t)mlp_argmax(z);
+    if (lane < 16 && valid) {
+      const uint8_t lv = (uint8_t)mlp_argmax(z);
+      changed = changed || ic.prec_out[t] != lv;
+      ic.prec_out[t] = lv;
+    }
   }
   __threadfence_block();
+  return __ballot(changed) != 0;
 }
 
 // Learned tile levels for the tiles of tile-rows [tr0, tr1).
-__device__ __forceinline__ void wave_infer(const TileView &A, const InferCtx &ic, int tr0, int tr1,
+__device__ __forceinline__ bool wave_infer(const TileView &A, const InferCtx &ic, int tr0, int tr1,
                                            int lane) {
-  if (tr0 >= tr1) return;
-  wave_infer_tiles(A, ic, A.tile_ptr[tr0], A.tile_ptr[tr1], lane);
+  if (tr0 >= tr1) return false;
+  return wave_infer_tiles(A, ic, A.tile_ptr[tr0], A.tile_ptr[tr1], lane);
 }
 
 // Deterministic sum over the 64 lanes of a wave (fixed xor tree); every lane

@@ -267,19 +267,23 @@ __device__ __forceinline__ void solver_spmv(const SolveArgs &a, const double *__
 // Learned levels for exactly the tiles the calling wave's SpMV reads: its
 // resident window, or its descriptors (whole tile-rows, and the pieces of
 // heavy tile-rows it holds; a heavy tile-row's tiles are inferred by the
-// waves that multiply them, never by its owner).
+// waves that multiply them, never by its owner).  Returns whether any of
+// them changed level (a resident window is restaged only then).
 template <bool RES>
-__device__ __forceinline__ void solver_infer(const SolveArgs &a, const InferCtx &ic, int tr0, int tr1,
+__device__ __forceinline__ bool solver_infer(const SolveArgs &a, const InferCtx &ic, int tr0, int tr1,
                                              int lane) {
   if constexpr (RES) {
-    wave_infer(a.A, ic, tr0, tr1, lane);
+    return wave_infer(a.A, ic, tr0, tr1, lane);
   } else {
     const int wid = solver_wave_id();
     const int d0 = __builtin_amdgcn_readfirstlane(a.wave_win[wid]);
     const int d1 = __builtin_amdgcn_readfirstlane(a.wave_win[wid + 1]);
+    bool changed = false;
     for (int d = d0; d < d1; ++d)
-      wave_infer_tiles(a.A, ic, __builtin_amdgcn_readfirstlane(a.win_desc[d].ta),
-                       __builtin_amdgcn_readfirstlane(a.win_desc[d].T1), lane);
+      changed = wave_infer_tiles(a.A, ic, __builtin_amdgcn_readfirstlane(a.win_desc[d].ta),
+                                 __builtin_amdgcn_readfirstlane(a.win_desc[d].T1), lane) ||
+                changed;
+    return changed;
   }
 }
 
@@ -429,8 +433,9 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
       // w = A r with gamma = r.r and delta = w.r of the wave's rows; new
       // levels (inference, or a rule update before the last sync) are staged
       // into a resident window first
-      if (infer && !force64) solver_infer<RES>(a, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
-      if ((infer && !force64) || restage) solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
+      bool relevel = false;
+      if (infer && !force64) relevel = solver_infer<RES>(a, infer_ctx(a, rr, rr, bb), tr0, tr1, lane);
+      if (relevel || restage) solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
       infer = false;
       restage = false;
       double gd[2] = {0.0, 0.0};
@@ -742,10 +747,8 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
     ++it;
     // v = A p, v.rh
     double vr[1] = {0.0};
-    if (infer && !force64) {
-      solver_infer<RES>(a, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane);
+    if (infer && !force64 && solver_infer<RES>(a, infer_ctx(a, pp_last, rr, bb), tr0, tr1, lane))
       solver_stage<MODE>(a, ctx, tr0, tr1, a.seg_p, lane);
-    }
     infer = false;
     solver_spmv<MODE, mixed>(a, a.p, a.p32, a.seg_p, lowv ? 2 : force64, ctx, lane, &cnt,
       [&](const int i, const double vv) {
