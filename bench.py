@@ -68,6 +68,35 @@ def _line(metric, value, unit, args, ms, config, **extra):
     print(json.dumps(d), flush=True)
 
 
+# Vector bytes per row and iteration besides the SpMV formula's x read and y
+# write: CG's update reads r, p, s, x and writes p, s, x, r, its SpMV epilogue
+# reads r; BiCGSTAB's five vector passes (v and r^ in the first SpMV's
+# epilogue, s = r - alpha v, s in the second epilogue, x and r, p).  FP32 work
+# vectors under the mixed policies, x always FP64.
+_VEC_BYTES = {("cg", False): 72, ("cg", True): 44, ("bicgstab", False): 128, ("bicgstab", True): 72}
+
+
+def _iteration_roofline(tm, res, method, levels=None):
+    """Modeled HBM bytes per iteration (the SpMV formula at the tile levels
+    the mixed solve ran with -- `levels`, else the handle's after the solve --
+    or all FP64, plus the vector passes) and the rate they imply at each
+    solve's measured kernel time per iteration."""
+    import millefeuille as mf
+    spmvs = 2 if method == "bicgstab" else 1
+    out = {}
+    for pol, r in res.items():
+        mixed = pol != "fp64" and not r["fell_back"]
+        lv = (tm.get_precision() if levels is None else levels) if mixed else None
+        b_spmv = mf.spmv_bytes(tm, levels=lv, x_bytes=4 if mixed else 8)
+        b_it = spmvs * b_spmv + _VEC_BYTES[(method, mixed)] * tm.rowA
+        us = 1e3 * r["kernel_ms"] / max(r["iterations"], 1)
+        gbs = b_it / (us * 1e-6) / 1e9
+        out[pol] = {"bytes_per_iteration": int(b_it), "us_per_iteration": round(us, 2),
+                    "achieved_gbs": round(gbs, 1), "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 4),
+                    "levels": "per tile, after the solve" if mixed else "all FP64"}
+    return out
+
+
 def _timed_solves(tm, solve, policies, args, n, rp, ci, va, b, dx):
     """Warm-up, then args.steps solves per policy alternated step by step;
     mean wall ms per solve, the last report and the FP64 relative residual of
@@ -124,6 +153,7 @@ def other_workload(args):
                "n": n, "nnz": int(rp[-1]), "tiles": tm.tilenum}
         _line("time-to-1e-10 residual (FP64-checked): mixed vs FP64 single-kernel BiCGSTAB",
               res["learned"]["ms"], "ms", args, res["learned"]["ms"], cfg, solves=res,
+              iteration_roofline=_iteration_roofline(tm, res, "bicgstab"),
               speedup_vs_fp64=round(res["fp64"]["ms"] / res["learned"]["ms"], 3))
     elif args.workload in ("powerlaw4m", "powerlaw1m"):
         big = args.workload == "powerlaw4m"
@@ -147,6 +177,7 @@ def other_workload(args):
                "n": n, "nnz": int(rp[-1]), "tiles": tm.tilenum}
         _line("time-to-1e-10 residual (FP64-checked): learned vs rule policy, single-kernel CG",
               res["learned"]["ms"], "ms", args, res["learned"]["ms"], cfg, solves=res,
+              iteration_roofline=_iteration_roofline(tm, res, "cg"),
               speedup_vs_rule=round(res["rule"]["ms"] / res["learned"]["ms"], 3),
               speedup_vs_fp64=round(res["fp64"]["ms"] / res["learned"]["ms"], 3))
     else:  # knn512
@@ -257,6 +288,7 @@ def main():
         rep = solve(policy)
         t_mixed.append(time.perf_counter() - t1)
         x = dx.get()[:n]
+        lv_mixed = tm.get_precision()  # the levels the mixed solve ended with
         t2 = time.perf_counter()
         rep64 = solve("fp64")
         t_fp64.append(time.perf_counter() - t2)
@@ -368,6 +400,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu_baseline,
+        "iteration_roofline": _iteration_roofline(
+            tm, {policy: {"kernel_ms": rep["time_ms"], "iterations": rep["iterations"],
+                          "fell_back": rep["fell_back"]},
+                 "fp64": {"kernel_ms": rep64["time_ms"], "iterations": rep64["iterations"],
+                          "fell_back": 0}}, "cg", levels=lv_mixed),
     }
     if not (rel_cpu <= 1e-10 and rel64 <= 1e-10):
         out["error"] = "FP64-checked residual above 1e-10"

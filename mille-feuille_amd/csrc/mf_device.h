@@ -1425,17 +1425,26 @@ __device__ __forceinline__ void rowgroup_passes(const TileView &A, int tr0, int 
 __device__ __forceinline__ bool wave_infer_tiles(const TileView &A, const InferCtx &ic, int t0, int t1,
                                                  int lane) {
   bool changed = false;
+  if (t0 >= t1) return false;
+  MlpRegs wr;
+  mlp_load(ic.w, lane, wr);
+  // features of the next batch are loaded while this one is evaluated
+  int t = t0 + (lane & 15);
+  bool valid = t < t1;
+  half4_t f = infer_features(ic, valid ? t : t0, (lane >= 16 && valid) ? A.tile_col[t] : 0, lane, valid);
   for (int g = t0; g < t1; g += 16) {
-    const int t = g + (lane & 15);
-    const bool valid = t < t1;
-    const int tt = valid ? t : t0;
-    const half4_t f = infer_features(ic, tt, (lane >= 16 && valid) ? A.tile_col[tt] : 0, lane, valid);
-    const float4_t z = mlp_eval16(ic.w, lane, f);
+    const int tn = g + 16 + (lane & 15);
+    const bool vn = tn < t1;
+    const half4_t fn = infer_features(ic, vn ? tn : t0, (lane >= 16 && vn) ? A.tile_col[tn] : 0, lane, vn);
+    const float4_t z = mlp_eval16r(wr, f);
+    f = fn;
     if (lane < 16 && valid) {
       const uint8_t lv = (uint8_t)mlp_argmax(z);
       changed = changed || ic.prec_out[t] != lv;
       ic.prec_out[t] = lv;
     }
+    t = tn;
+    valid = vn;
   }
   __threadfence_block();
   return __ballot(changed) != 0;

@@ -57,8 +57,7 @@ __device__ __forceinline__ half4_t relu_half(float4_t v) {
 // 4*(lane/16)..+3 of tile lane%16; zero in lanes 32-63).  Returns the D block
 // of layer 3: lanes 0-15 hold the 4 logits of tile lane%16.  The operands are
 // read from `w` (L1-resident after the first batch) right where they are
-// used, which keeps the evaluation within the register budget of the solver
-// kernels that call it.
+// used (mf_mlp_forward; the solvers hold them in registers, mlp_eval16r).
 __device__ __forceinline__ float4_t mlp_eval16(const MlpDevice *__restrict__ w, int lane, half4_t f) {
   // The scheduling barriers keep each stage's operand loads next to their
   // MFMAs (hoisted together they would need ~36 more registers at once).
@@ -86,6 +85,40 @@ __device__ __forceinline__ int mlp_argmax(float4_t z) {
   if (z.z > m) { m = z.z; c = 2; }
   if (z.w > m) { c = 3; }
   return c;
+}
+
+// The MLP's operands of one lane, loaded once into registers for a run of
+// batches (the solvers' in-kernel inference: about 36 VGPRs, live only while
+// the wave evaluates its tiles, outside the SpMV loops).
+struct MlpRegs {
+  half4_t a[8];
+  float4_t c1[2], c2[2], c3;
+};
+
+__device__ __forceinline__ void mlp_load(const MlpDevice *__restrict__ w, int lane, MlpRegs &r) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r.a[k] = w->a[k][lane];
+  r.c1[0] = w->c1[0][lane];
+  r.c1[1] = w->c1[1][lane];
+  r.c2[0] = w->c2[0][lane];
+  r.c2[1] = w->c2[1][lane];
+  r.c3 = w->c3[lane];
+}
+
+// mlp_eval16 with the operands in registers: the eight MFMAs wait only on
+// each other, not on operand loads.
+__device__ __forceinline__ float4_t mlp_eval16r(const MlpRegs &w, half4_t f) {
+  const float4_t h1a = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a[0], f, w.c1[0], 0, 0, 0);
+  const float4_t h1b = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a[1], f, w.c1[1], 0, 0, 0);
+  const half4_t g0 = relu_half(h1a), g1 = relu_half(h1b);
+  float4_t h2a = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a[2], g0, w.c2[0], 0, 0, 0);
+  h2a = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a[3], g1, h2a, 0, 0, 0);
+  float4_t h2b = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a[4], g0, w.c2[1], 0, 0, 0);
+  h2b = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a[5], g1, h2b, 0, 0, 0);
+  const half4_t q0 = relu_half(h2a), q1 = relu_half(h2b);
+  float4_t z = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a[6], q0, w.c3, 0, 0, 0);
+  z = __builtin_amdgcn_mfma_f32_16x16x16f16(w.a[7], q1, z, 0, 0, 0);
+  return z;
 }
 
 // In-kernel inference context: the MLP, the per-tile static features (8
