@@ -1153,6 +1153,26 @@ void build_schedule(const mf_tiled *t, int nwaves, HostSchedule &hsch) {
       while (r < tm && pre[r] < target) ++r;
       b[w] = std::max(r, b[w - 1]);
     }
+    // The vector passes take a wave's rows 64 * kUpd at a time, so a wave
+    // one tile-row over a multiple of 4 * kUpd tile-rows pays a whole extra
+    // pass (about 2048 slots' worth), which the additive cost above does not
+    // see.  An equal split of tile-rows is kept instead when its busiest wave,
+    // counted that way, has less work (regular matrices whose boundary
+    // tile-rows are lighter than the rest, such as stencils).
+    auto busiest = [&](const std::vector<int> &bb) {
+      double m = 0.0;
+      for (int w = 0; w < nwaves; ++w) {
+        double c = 0.0;
+        for (int q = bb[w]; q < bb[w + 1]; ++q)
+          if (!heavy[q]) c += (double)slots(q) + 8.0 * tiles(q);
+        const int passes = (16 * (bb[w + 1] - bb[w]) + 64 * kUpd - 1) / (64 * kUpd);
+        m = std::max(m, c + 2048.0 * passes);
+      }
+      return m;
+    };
+    std::vector<int> even(nwaves + 1);
+    for (int w = 0; w <= nwaves; ++w) even[w] = (int)((long long)tm * w / nwaves);
+    if (busiest(even) < busiest(b)) b = even;
   }
   // descriptors of each wave's non-heavy tile-rows, and the wave's load
   std::vector<std::vector<WaveDesc>> per(nwaves);
