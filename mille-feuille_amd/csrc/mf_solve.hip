@@ -367,6 +367,14 @@ __device__ __forceinline__ double restart_rows(const SolveArgs &a, int lane, boo
 // Vectors: FP64 r, q (w), p, s / FP32 r32, q32 (w), p32, s32.  With resident
 // windows w never goes to memory: the update reads it from the wave's row
 // accumulators, where the SpMV left it.
+// Store of a vector pass's result that writes through the XCD's L2 (an
+// agent-scope store): the grid barrier's release then has few dirty L2 lines
+// to write back (at 128^3 the FP64 CG iteration went from 59.4 to 54.7 us).
+template <typename T>
+__device__ __forceinline__ void wt_store(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int MODE, bool RES>
 __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
   __shared__ double lds[kMaxGridSum * kWavesPerBlock];
@@ -578,10 +586,10 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           const float rf = (float)(rv[u] - alpha * (double)sf);
           rn[u] = rf;
           if (i < row1) {
-            a.p32[i] = pf;
-            a.s32[i] = sf;
-            a.x[i] = x_[u] + alpha * (double)pf;
-            a.r32[i] = rf;
+            wt_store(a.p32 + i, pf);
+            wt_store(a.s32 + i, sf);
+            wt_store(a.x + i, x_[u] + alpha * (double)pf);
+            wt_store(a.r32 + i, rf);
           }
         }
       } else {
@@ -607,10 +615,10 @@ __global__ __launch_bounds__(kBlock) void k_cg(SolveArgs a) {
           const double sv = q_[u] + beta * s_[u];
           rn[u] = rv[u] - alpha * sv;
           if (i < row1) {
-            a.p[i] = pv;
-            a.s[i] = sv;
-            a.x[i] = x_[u] + alpha * pv;
-            a.r[i] = rn[u];
+            wt_store(a.p + i, pv);
+            wt_store(a.s + i, sv);
+            wt_store(a.x + i, x_[u] + alpha * pv);
+            wt_store(a.r + i, rn[u]);
           }
         }
       }
@@ -785,7 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
         for (int u = 0; u < kUpd; ++u) {
           const int i = base0 + 64 * u + lane;
           const float sf = (float)((double)r_[u] - alpha * (double)v_[u]);
-          if (i < row1) a.s32[i] = sf;
+          if (i < row1) wt_store(a.s32 + i, sf);
           so[u] = s_[u];
           sn[u] = sf;
         }
@@ -803,7 +811,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
         for (int u = 0; u < kUpd; ++u) {
           const int i = base0 + 64 * u + lane;
           const double sv = r_[u] - alpha * v_[u];
-          if (i < row1) a.s[i] = sv;
+          if (i < row1) wt_store(a.s + i, sv);
           so[u] = s_[u];
           sn[u] = sv;
         }
@@ -863,14 +871,14 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
       for (int u = 0; u < kUpd; ++u) {
         const int i = base + 64 * u;
         if (i < row1) {
-          a.x[i] = x_[u] + (alpha * p_[u] + omega * s_[u]);
+          wt_store(a.x + i, x_[u] + (alpha * p_[u] + omega * s_[u]));
           double rv = s_[u] - omega * t_[u];
           if (lowv) {
             const float rf = (float)rv;
-            a.r32[i] = rf;
+            wt_store(a.r32 + i, rf);
             rv = rf;
           } else {
-            a.r[i] = rv;
+            wt_store(a.r + i, rv);
           }
           nw[0] += rv * h_[u];
           nw[1] += rv * rv;
@@ -946,7 +954,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
           const int i = base0 + 64 * u + lane;
           const float pf =
               (float)((double)r_[u] + beta * ((double)p_[u] - omega * (double)v_[u]));
-          if (i < row1) a.p32[i] = pf;
+          if (i < row1) wt_store(a.p32 + i, pf);
           po[u] = p_[u];
           pn[u] = pf;
         }
@@ -964,7 +972,7 @@ __global__ __launch_bounds__(kBlock) void k_bicgstab(SolveArgs a) {
         for (int u = 0; u < kUpd; ++u) {
           const int i = base0 + 64 * u + lane;
           const double pv = r_[u] + beta * (p_[u] - omega * v_[u]);
-          if (i < row1) a.p[i] = pv;
+          if (i < row1) wt_store(a.p + i, pv);
           po[u] = p_[u];
           pn[u] = pv;
         }
