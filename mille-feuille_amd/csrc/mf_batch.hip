@@ -14,8 +14,9 @@
 // at a block barrier; there is no grid barrier, so blocks need not be
 // co-resident and the launch is an ordinary one.  Precision follows the
 // handle's current tile levels (FP64, or the FIXED / ERRBOUND levels set
-// before the launch) with FP64 scalars, FP64 x, periodic FP64 true-residual
-// recomputation and FP64 fallback on stagnation.
+// before the launch) with FP64 scalars, FP64 x, FP32 work vectors under mixed
+// levels, periodic FP64 true-residual recomputation and FP64 fallback on
+// stagnation.
 //
 // Timing: each item records the realtime clock (100 MHz) at start and end;
 // a system's time is its last end minus its first start.
@@ -105,14 +106,28 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cg(BatchArgs a) {
       p[i] = 0.0;
       p32[i] = 0.0f;
     }
+    // Mixed items keep r, p and q in FP32 (r32 and q32 over the first halves
+    // of r and q) while their tiles are mixed; x, the scalars and the residual
+    // recomputations stay FP64, and FP64 tiles multiply p32 in FP64
+    // arithmetic.  A fallback to FP64 tiles restarts in FP64 vectors.
+    const bool mixed = S.mixed != 0;
+    bool lowv = mixed;
+    float *r32 = (float *)r, *q32 = (float *)q;
     // r = b - A x, p = r
     double rr0 = 0.0, bb0 = 0.0;
     wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
       [&](const int i, const double ax) {
-        const double bv = b[i], rv = bv - ax;
-        r[i] = rv;
-        p[i] = rv;
-        p32[i] = (float)rv;
+        const double bv = b[i];
+        double rv = bv - ax;
+        if (lowv) {
+          const float rf = (float)rv;
+          r32[i] = rf;
+          p32[i] = rf;
+          rv = rf;
+        } else {
+          r[i] = rv;
+          p[i] = rv;
+        }
         rr0 += rv * rv;
         bb0 += bv * bv;
       });
@@ -122,19 +137,25 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cg(BatchArgs a) {
     double rr_ref = rr;
     int it = 0, it_ref = 0, refresh = 0;
     int force64 = S.mixed ? 0 : 1;
-    const bool mixed = S.mixed != 0;
     bool restart = false;
     bool done = bb == 0.0 || rr <= stop2;
     __syncthreads();
     while (!done) {
       if (restart) {
+        if (lowv && force64) lowv = false;
         double tr0v = 0.0;
         wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
           [&](const int i, const double ax) {
-            const double rv = b[i] - ax;
-            r[i] = rv;
-            p[i] = rv;
-            p32[i] = (float)rv;
+            double rv = b[i] - ax;
+            if (lowv) {
+              const float rf = (float)rv;
+              r32[i] = rf;
+              p32[i] = rf;
+              rv = rf;
+            } else {
+              r[i] = rv;
+              p[i] = rv;
+            }
             tr0v += rv * rv;
           });
         rr = block_sum(tr0v, lds);
@@ -147,10 +168,16 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cg(BatchArgs a) {
       }
       ++it;
       double pq0 = 0.0;
-      wave_spmv<kPerTile, false, 3, false, true>(A, wave_desc(A, tr0, tr1), p, p32, nullptr, force64, L,
-                                                 lane, &cnt, [&](const int i, const double qv) {
-          q[i] = qv;
-          pq0 += p[i] * qv;
+      wave_spmv<kPerTile, false, 3, false, true>(A, wave_desc(A, tr0, tr1), p, p32, nullptr,
+                                                 lowv ? 2 : force64, L, lane, &cnt,
+                                                 [&](const int i, const double qv) {
+          if (lowv) {
+            q32[i] = (float)qv;
+            pq0 += (double)p32[i] * qv;
+          } else {
+            q[i] = qv;
+            pq0 += p[i] * qv;
+          }
         });
       const double pq = block_sum(pq0, lds);
       if (mixed && !force64 && !(pq > 0.0 && isfinite(pq))) {
@@ -161,11 +188,20 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cg(BatchArgs a) {
       if (!(pq != 0.0) || !isfinite(pq)) break;
       const double alpha = rr / pq;
       double rn0 = 0.0;
-      for (int i = row0 + lane; i < row1; i += 64) {
-        x[i] += alpha * p[i];
-        const double rv = r[i] - alpha * q[i];
-        r[i] = rv;
-        rn0 += rv * rv;
+      if (lowv) {
+        for (int i = row0 + lane; i < row1; i += 64) {
+          x[i] += alpha * (double)p32[i];
+          const float rf = (float)((double)r32[i] - alpha * (double)q32[i]);
+          r32[i] = rf;
+          rn0 += (double)rf * rf;
+        }
+      } else {
+        for (int i = row0 + lane; i < row1; i += 64) {
+          x[i] += alpha * p[i];
+          const double rv = r[i] - alpha * q[i];
+          r[i] = rv;
+          rn0 += rv * rv;
+        }
       }
       double rr_new = block_sum(rn0, lds);
       const bool converged = rr_new <= stop2;
@@ -175,8 +211,14 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cg(BatchArgs a) {
         double tv = 0.0;
         wave_spmv<kAll64, false>(A, tr0, tr1, x, nullptr, nullptr, 0, L, lane, nullptr,
           [&](const int i, const double ax) {
-            const double rv = b[i] - ax;
-            r[i] = rv;
+            double rv = b[i] - ax;
+            if (lowv) {
+              const float rf = (float)rv;
+              r32[i] = rf;
+              rv = rf;
+            } else {
+              r[i] = rv;
+            }
             tv += rv * rv;
           });
         const double tr = block_sum(tv, lds);
@@ -198,10 +240,11 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cg(BatchArgs a) {
       }
       const double beta = rr_new / rr;
       rr = rr_new;
-      for (int i = row0 + lane; i < row1; i += 64) {
-        const double pv = r[i] + beta * p[i];
-        p[i] = pv;
-        p32[i] = (float)pv;
+      if (lowv) {
+        for (int i = row0 + lane; i < row1; i += 64)
+          p32[i] = (float)((double)r32[i] + beta * (double)p32[i]);
+      } else {
+        for (int i = row0 + lane; i < row1; i += 64) p[i] = r[i] + beta * p[i];
       }
       __syncthreads();
     }
