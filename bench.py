@@ -228,6 +228,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--policy", default="learned")
+    ap.add_argument("--refresh-ratio", type=float, default=0.01,
+                    help="mixed CG on the Laplacian: FP64 residual recomputation when the "
+                         "recursive residual fell by this factor (library default 0.1)")
+    ap.add_argument("--refresh-every", type=int, default=128,
+                    help="mixed CG on the Laplacian: ... or after this many iterations "
+                         "(library default 64)")
     ap.add_argument("--check-interval", type=int, default=32,
                     help="iterations between policy evaluations (learned inference)")
     ap.add_argument("--cpu-iters", type=int, default=0,
@@ -265,11 +271,14 @@ def main():
         from millefeuille import policy as pol
         pol.install(tm, pol.load_or_train())
         policy_note = (f"learned (MLP 8-32-32-4, policy_mlp.f32, in-kernel inference every "
-                       f"{args.check_interval} iterations)")
+                       f"{args.check_interval} iterations; FP64 residual recomputation when the "
+                       f"recursive residual fell {1 / args.refresh_ratio:g}x or every "
+                       f"{args.refresh_every} iterations)")
 
     def solve(pol):
         dx.zero()
-        return tm.cg(db, dx, tol=1e-10, policy=pol, check_interval=args.check_interval)
+        return tm.cg(db, dx, tol=1e-10, policy=pol, check_interval=args.check_interval,
+                     refresh_ratio=args.refresh_ratio, refresh_every=args.refresh_every)
 
     # --- warm-up, then mixed and FP64 solves alternated step by step, so
     # both see the same device state; each solve timed on its own --------
