@@ -146,11 +146,14 @@ def other_workload(args):
         def solve(pol):
             dx.zero()
             return tm.bicgstab(db, dx, allow_not_converged=True, tol=1e-10, policy=pol,
-                               check_interval=args.check_interval)
+                               check_interval=args.check_interval,
+                               refresh_ratio=args.refresh_ratio, refresh_every=args.refresh_every)
         res = _timed_solves(tm, solve, ["learned", "errbound", "fp64"], args, n, rp, ci, va, b, dx)
         cfg = {"workload": "BiCGSTAB, 3D 7-point upwind convection-diffusion 100^3, cell Peclet 0.5, "
                            "velocity (1,0.5,0.25), b = A x*, x* ~ U[-1,1] seed 3, x0 = 0, tol 1e-10",
-               "n": n, "nnz": int(rp[-1]), "tiles": tm.tilenum}
+               "n": n, "nnz": int(rp[-1]), "tiles": tm.tilenum,
+               "mixed_refresh": f"FP64 residual after a {1 / args.refresh_ratio:g}x drop or every "
+                                f"{args.refresh_every} iterations"}
         _line("time-to-1e-10 residual (FP64-checked): mixed vs FP64 single-kernel BiCGSTAB",
               res["learned"]["ms"], "ms", args, res["learned"]["ms"], cfg, solves=res,
               iteration_roofline=_iteration_roofline(tm, res, "bicgstab"),
@@ -229,11 +232,12 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--policy", default="learned")
     ap.add_argument("--refresh-ratio", type=float, default=0.01,
-                    help="mixed CG on the Laplacian: FP64 residual recomputation when the "
-                         "recursive residual fell by this factor (library default 0.1)")
+                    help="mixed CG on the Laplacian and mixed BiCGSTAB: FP64 residual "
+                         "recomputation when the recursive residual fell by this factor "
+                         "(library default 0.1)")
     ap.add_argument("--refresh-every", type=int, default=128,
-                    help="mixed CG on the Laplacian: ... or after this many iterations "
-                         "(library default 64)")
+                    help="mixed CG on the Laplacian and mixed BiCGSTAB: ... or after this "
+                         "many iterations (library default 64)")
     ap.add_argument("--check-interval", type=int, default=32,
                     help="iterations between policy evaluations (learned inference)")
     ap.add_argument("--cpu-iters", type=int, default=0,
