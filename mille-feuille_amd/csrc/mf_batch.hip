@@ -1,12 +1,15 @@
 // mf_batch.hip -- many independent systems, each with several right-hand
 // sides, solved by CG in one persistent launch captured in a hipGraph.
 //
-// A work item is one (system, right-hand side) pair.  Items are ordered
-// largest system first (the longest-processing-time rule) and handed out by a
-// global queue: every block takes the next item, solves it with its 8 waves
-// and takes another, until the queue is empty.  An item's right-hand sides
-// sit next to each other in the queue, so the blocks solving them run at the
-// same time and share the system's tiles through L2 and the Infinity Cache.
+// A work item is one (system, right-hand side) pair.  The systems are split
+// over 8 queues, one per XCD (blocks are dispatched round-robin, block b on
+// XCD b % 8), by the longest-processing-time rule on their work (entries
+// times right-hand sides); inside a queue they run largest first, with a
+// system's right-hand sides next to each other.  Every block takes the next
+// item of its XCD's queue, solves it with its waves and takes another; when
+// that queue is empty it takes items from the other queues in turn.  So the
+// blocks solving one system's right-hand sides run at the same time on one
+// XCD and read the system's tiles through that XCD's L2.
 //
 // Inside a block the solve is the single-system memory-form CG: the 8 waves
 // own nnz-balanced tile-row ranges of the system, the SpMV is the streamed
@@ -45,9 +48,10 @@ struct BatchSys {
 
 struct BatchArgs {
   const BatchSys *sys;
-  const int2 *items;  // (system, right-hand side), largest system first
+  const int2 *items;  // (system, right-hand side), queue by queue
+  const int *qoff;    // kQueues + 1: items of queue q at [qoff[q], qoff[q+1])
   int nitems;
-  int *head;          // work-queue head
+  int *head;          // kQueues work-queue heads
   double *work;       // per block: r, p, q (npad_max doubles each), p32
   long long stride;   // doubles per block
   int *iters;
@@ -60,11 +64,13 @@ struct BatchArgs {
   int refresh_every;
 };
 
+constexpr int kQueues = 8;  // one per XCD
+
 __device__ __forceinline__ unsigned long long realtime() { return __builtin_amdgcn_s_memrealtime(); }
 
 __global__ void k_batch_reset(int *head, int nitems, int *iters, double *relres) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *head = 0;
+  if (i < kQueues) head[i] = 0;
   if (i < nitems) {
     iters[i] = 0;
     relres[i] = -1.0;
@@ -81,13 +87,28 @@ __global__ __launch_bounds__(BLOCK) void k_batch_cg(BatchArgs a) {
   WaveLds &L = wl[wv];
   double *r = a.work + (long long)blockIdx.x * a.stride;
   PrecCount cnt;
+  int q = blockIdx.x % kQueues, tried = 0;  // this XCD's queue first
   for (;;) {
-    if (threadIdx.x == 0) item_s = atomicAdd(a.head, 1);
+    if (threadIdx.x == 0) {
+      // next item of queue q; an empty queue passes to the next one, and
+      // after all kQueues are empty the block ends (every head only grows)
+      int it = -1;
+      while (tried < kQueues) {
+        const int k = atomicAdd(&a.head[q], 1);
+        if (a.qoff[q] + k < a.qoff[q + 1]) {
+          it = a.qoff[q] + k;
+          break;
+        }
+        q = (q + 1) % kQueues;
+        ++tried;
+      }
+      item_s = it;
+    }
     __syncthreads();
     // item, system and wave range are wave-uniform: scalar registers
     const int item = __builtin_amdgcn_readfirstlane(item_s);
     __syncthreads();
-    if (item >= a.nitems) break;
+    if (item < 0) break;
     const unsigned long long t_start = realtime();
     const int sidx = __builtin_amdgcn_readfirstlane(a.items[item].x);
     const int rhs = __builtin_amdgcn_readfirstlane(a.items[item].y);
@@ -352,16 +373,28 @@ extern "C" mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs, c
     hs[i].npad = systems[i]->dev.npad;
     hs[i].mixed = opt.policy == MF_POLICY_FP64 ? 0 : 1;
   }
-  // items, largest system first
+  // systems over the queues: largest first, each to the least loaded queue
+  // (work = entries; every system has nrhs items); items queue by queue
   std::vector<int> order(nsys);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
                    [&](int u, int v) { return systems[u]->csrsize > systems[v]->csrsize; });
+  std::vector<std::vector<int>> qsys(kQueues);
+  std::vector<double> qload(kQueues, 0.0);
+  for (int s : order) {
+    const int q = (int)(std::min_element(qload.begin(), qload.end()) - qload.begin());
+    qsys[q].push_back(s);
+    qload[q] += (double)systems[s]->csrsize + 1.0;
+  }
   const int nitems = nsys * nrhs;
   std::vector<int2> hitems;
+  std::vector<int> hqoff(kQueues + 1, 0);
   hitems.reserve(nitems);
-  for (int s : order)
-    for (int j = 0; j < nrhs; ++j) hitems.push_back(make_int2(s, j));
+  for (int q = 0; q < kQueues; ++q) {
+    for (int s : qsys[q])
+      for (int j = 0; j < nrhs; ++j) hitems.push_back(make_int2(s, j));
+    hqoff[q + 1] = (int)hitems.size();
+  }
 
   int ncu = 0, nb = 0;
   MF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -376,23 +409,26 @@ extern "C" mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs, c
 
   BatchSys *d_sys = nullptr;
   int2 *d_items = nullptr;
-  int *d_head = nullptr, *d_iters = nullptr;
+  int *d_head = nullptr, *d_iters = nullptr, *d_qoff = nullptr;
   double *d_rel = nullptr, *d_work = nullptr;
   unsigned long long *d_ts = nullptr;
   MF_TRY(dalloc((void **)&d_sys, sizeof(BatchSys) * nsys));
   MF_TRY(dalloc((void **)&d_items, sizeof(int2) * nitems));
-  MF_TRY(dalloc((void **)&d_head, sizeof(int)));
+  MF_TRY(dalloc((void **)&d_head, sizeof(int) * kQueues));
+  MF_TRY(dalloc((void **)&d_qoff, sizeof(int) * (kQueues + 1)));
   MF_TRY(dalloc((void **)&d_iters, sizeof(int) * nitems));
   MF_TRY(dalloc((void **)&d_rel, sizeof(double) * nitems));
   MF_TRY(dalloc((void **)&d_ts, sizeof(unsigned long long) * 2 * nitems));
   MF_TRY(dalloc((void **)&d_work, sizeof(double) * stride * G));
   MF_HIP(hipMemcpyAsync(d_sys, hs.data(), sizeof(BatchSys) * nsys, hipMemcpyHostToDevice, st));
   MF_HIP(hipMemcpyAsync(d_items, hitems.data(), sizeof(int2) * nitems, hipMemcpyHostToDevice, st));
+  MF_HIP(hipMemcpyAsync(d_qoff, hqoff.data(), sizeof(int) * (kQueues + 1), hipMemcpyHostToDevice, st));
   MF_HIP(hipStreamSynchronize(st));
 
   BatchArgs args;
   args.sys = d_sys;
   args.items = d_items;
+  args.qoff = d_qoff;
   args.nitems = nitems;
   args.head = d_head;
   args.work = d_work;
@@ -408,7 +444,7 @@ extern "C" mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs, c
 
   // capture: queue reset (a kernel, not a memset node) + the persistent solve
   MF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  k_batch_reset<<<(nitems + 255) / 256, 256, 0, st>>>(d_head, nitems, d_iters, d_rel);
+  k_batch_reset<<<(std::max(nitems, kQueues) + 255) / 256, 256, 0, st>>>(d_head, nitems, d_iters, d_rel);
   if (nwv == 4) k_batch_cg<256><<<G, 256, 0, st>>>(args);
   else if (nwv == 8) k_batch_cg<512><<<G, 512, 0, st>>>(args);
   else k_batch_cg<1024><<<G, 1024, 0, st>>>(args);
