@@ -383,6 +383,10 @@ def main():
     if not args.skip_cpu_baseline:
         cpu_baseline = _cpu_baseline(args, rp, ci, va, b, xs, n, rep64)
 
+    # --- the persistent solver against the launch-per-operation one --------
+    launches = _single_vs_multikernel(tm, db, dx, rp, ci, va, b,
+                                      f"3D 7-point Laplacian {args.grid}^3")
+
 
     out = {
         "metric": "time-to-1e-10 residual (FP64-checked): learned/mixed vs FP64 CG; SpMV HBM GB/s",
@@ -413,6 +417,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu_baseline,
+        "single_vs_multikernel_fp64_cg": launches,
         "iteration_roofline": _iteration_roofline(
             tm, {policy: {"kernel_ms": rep["time_ms"], "iterations": rep["iterations"],
                           "fell_back": rep["fell_back"]},
@@ -475,6 +480,49 @@ def _cpu_baseline(args, rp, ci, va, b, xs, n, rep64):
                        + f"; oracle tiled SpMV (blockspmv_cpu loop with Tile_csr_Col), median of 5: "
                          f"{spmv[1]:.1f} ms on 1 thread, {spmv[ncpu]:.1f} ms on {ncpu} threads"),
             "spmv_ms_1thread": round(spmv[1], 2), "spmv_ms_all_threads": round(spmv[ncpu], 2)}
+
+
+def _single_vs_multikernel(tm, db, dx, rp, ci, va, b, label, small_grid=32, runs=3):
+    """FP64 CG to 1e-10 by the persistent single-kernel solver and by the
+    launch-per-operation solver (the reference's multi-kernel comparator,
+    cg_solve_reduce, src/main-cg.cu:245-279: one launch per SpMV, dot product
+    and vector update, the scalars read back on the host every iteration), on
+    the benchmark matrix and on a small grid where the launch gaps dominate.
+    Wall ms per solve (mean of `runs` after one warm-up), kernel-region ms
+    (HIP events), iterations and the oracle's FP64 residual of each."""
+    import millefeuille as mf
+    import oracle
+    from millefeuille import gen
+    rp2, ci2, va2 = gen.laplace3d_scaled(small_grid, seed=2)
+    b2 = gen.uniform_vector(len(rp2) - 1, 2)
+    tm2 = mf.TiledMatrix(rp2, ci2, va2)
+    db2, dx2 = tm2.vector(b2), tm2.vector()
+    out = {}
+    for name, t, d_b, d_x, (a_rp, a_ci, a_va, a_b) in (
+            (label, tm, db, dx, (rp, ci, va, b)),
+            (f"3D 7-point Laplacian {small_grid}^3", tm2, db2, dx2, (rp2, ci2, va2, b2))):
+        n = len(a_rp) - 1
+        row = {"n": n}
+        for kind, fn in (("single_kernel", t.cg), ("multi_kernel", t.cg_multikernel)):
+            d_x.zero()
+            fn(d_b, d_x, tol=1e-10, policy="fp64")
+            wall = []
+            for _ in range(runs):
+                d_x.zero()
+                t1 = time.perf_counter()
+                r = fn(d_b, d_x, tol=1e-10, policy="fp64")
+                wall.append(time.perf_counter() - t1)
+            rel = oracle.rel_residual(a_rp, a_ci, a_va, a_b, d_x.get()[:n])
+            ms = 1e3 * sum(wall) / runs
+            row[kind] = {"ms": round(ms, 3), "kernel_ms": round(r["time_ms"], 3),
+                         "iterations": r["iterations"],
+                         "us_per_iteration": round(1e3 * ms / max(r["iterations"], 1), 2),
+                         "rel_residual_fp64_cpu": rel}
+        row["multi_over_single"] = round(row["multi_kernel"]["ms"] / row["single_kernel"]["ms"], 3)
+        log(f"single vs multi-kernel FP64 CG, {name}: {row}")
+        out[name] = row
+    _release(db2, dx2, tm2)
+    return out
 
 
 def _release(*objs):
