@@ -149,6 +149,19 @@ def other_workload(args):
                                check_interval=args.check_interval,
                                refresh_ratio=args.refresh_ratio, refresh_every=args.refresh_every)
         res = _timed_solves(tm, solve, ["learned", "errbound", "fp64"], args, n, rp, ci, va, b, dx)
+        cpu_baseline = None
+        if not args.skip_cpu_baseline:
+            import oracle
+            tc = time.perf_counter()
+            _, it_c, rel_c = oracle.bicgstab(rp, ci, va, b, tol=1e-10, maxiter=20000)
+            cpu_s = time.perf_counter() - tc
+            cpu, ncpu, env = _cpu_info()
+            log(f"cpu ({cpu}): BiCGSTAB {it_c} iterations in {cpu_s:.2f} s, rel {rel_c:.2e}")
+            cpu_baseline = {"value": round(cpu_s * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                            "cpu_model": cpu, "usable_cores": ncpu, "omp_env": env,
+                            "sample": f"oracle FP64 BiCGSTAB (textbook, CSR SpMV, 1 thread) on this "
+                                      f"workload to 1e-10: {it_c} iterations in {cpu_s:.2f} s, "
+                                      f"rel {rel_c:.2e}"}
         cfg = {"workload": "BiCGSTAB, 3D 7-point upwind convection-diffusion 100^3, cell Peclet 0.5, "
                            "velocity (1,0.5,0.25), b = A x*, x* ~ U[-1,1] seed 3, x0 = 0, tol 1e-10",
                "n": n, "nnz": int(rp[-1]), "tiles": tm.tilenum,
@@ -156,7 +169,7 @@ def other_workload(args):
                                 f"{args.refresh_every} iterations"}
         _line("time-to-1e-10 residual (FP64-checked): mixed vs FP64 single-kernel BiCGSTAB",
               res["learned"]["ms"], "ms", args, res["learned"]["ms"], cfg, solves=res,
-              iteration_roofline=_iteration_roofline(tm, res, "bicgstab"),
+              iteration_roofline=_iteration_roofline(tm, res, "bicgstab"), cpu_baseline=cpu_baseline,
               speedup_vs_fp64=round(res["fp64"]["ms"] / res["learned"]["ms"], 3))
     elif args.workload in ("powerlaw4m", "powerlaw1m"):
         big = args.workload == "powerlaw4m"
@@ -386,6 +399,7 @@ def main():
     # --- the persistent solver against the launch-per-operation one --------
     launches = _single_vs_multikernel(tm, db, dx, rp, ci, va, b,
                                       f"3D 7-point Laplacian {args.grid}^3")
+    small = _small_2d_case()
 
 
     out = {
@@ -418,6 +432,7 @@ def main():
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu_baseline,
         "single_vs_multikernel_fp64_cg": launches,
+        "laplace2d_64": small,
         "iteration_roofline": _iteration_roofline(
             tm, {policy: {"kernel_ms": rep["time_ms"], "iterations": rep["iterations"],
                           "fell_back": rep["fell_back"]},
@@ -439,6 +454,47 @@ def _cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def _cpu_info():
+    """CPU model, the host cores this process may use, and the OpenMP
+    placement variables as set."""
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PLACES", "OMP_PROC_BIND")}
+    return _cpu_model(), len(os.sched_getaffinity(0)), env
+
+
+def _small_2d_case(runs=5):
+    """The reference's CPU-runnable case: 2D 5-point Laplacian on a 64 x 64
+    grid (n = 4096), b = A * ones, x0 = 0, tol 1e-10. FP64 CG on the GPU
+    (persistent kernel, wall ms per solve, mean of `runs` after one warm-up)
+    and the oracle's FP64 CG on one host thread, both FP64-checked."""
+    import millefeuille as mf
+    import oracle
+    from millefeuille import gen
+    rp, ci, va = gen.laplace2d(64)
+    n = len(rp) - 1
+    b = gen.csr_matvec(rp, ci, va, np.ones(n))
+    t = mf.TiledMatrix(rp, ci, va)
+    db, dx = t.vector(b), t.vector()
+    t.cg(db, dx, tol=1e-10, policy="fp64")
+    wall = []
+    for _ in range(runs):
+        dx.zero()
+        t1 = time.perf_counter()
+        r = t.cg(db, dx, tol=1e-10, policy="fp64")
+        wall.append(time.perf_counter() - t1)
+    rel_g = oracle.rel_residual(rp, ci, va, b, dx.get()[:n])
+    _release(db, dx, t)
+    tc = time.perf_counter()
+    _, it_c, rel_c = oracle.cg(rp, ci, va, b, tol=1e-10, maxiter=20000)
+    cpu_ms = (time.perf_counter() - tc) * 1e3
+    out = {"n": n, "nnz": int(rp[-1]),
+           "gpu_fp64_cg": {"ms": round(1e3 * sum(wall) / runs, 3), "kernel_ms": round(r["time_ms"], 3),
+                           "iterations": r["iterations"], "rel_residual_fp64_cpu": rel_g},
+           "cpu_oracle_fp64_cg_1thread": {"ms": round(cpu_ms, 3), "iterations": it_c,
+                                          "rel_residual": rel_c}}
+    log(f"2D 5-point Laplacian 64x64: {out}")
+    return out
 
 
 def _cpu_baseline(args, rp, ci, va, b, xs, n, rep64):
@@ -466,8 +522,7 @@ def _cpu_baseline(args, rp, ci, va, b, xs, n, rep64):
             oracle.blockspmv_tiled(ot, xs, nthreads=nt)
             reps.append((time.perf_counter() - tcs) * 1e3)
         spmv[nt] = float(np.median(reps))
-    cpu = _cpu_model()
-    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PLACES", "OMP_PROC_BIND")}
+    cpu, _, env = _cpu_info()
     log(f"cpu ({cpu}, {ncpu} usable cores, {env}): CG {it_cpu} iterations in {cpu_s:.2f} s "
         f"({per_it_ms:.1f} ms/iteration, 1 thread), rel {rel_c:.2e}; tiled spmv "
         f"{spmv[1]:.1f} ms (1 thread), {spmv[ncpu]:.1f} ms ({ncpu} threads)")
