@@ -224,7 +224,9 @@ typedef struct mf_solve_opts {
   int refresh_every;    /* mixed policies: FP64 residual check at least every
                            this many iterations; no progress since the last
                            check -> FP64 fallback and restart (0 -> 64) */
-  int blocks;           /* persistent grid size override (0 = occupancy) */
+  int blocks;           /* persistent grid size override (0: about 1024 rows
+                           per block, at least 4 blocks, at most one block of
+                           16 waves per CU) */
   int fixed_iters;      /* >0: run exactly this many iterations, no test
                            (reference timing mode, src/Mille-feuille-cg.h:426) */
   int form;             /* solver SpMV form: MF_FORM_MEMORY (every SpMV reads

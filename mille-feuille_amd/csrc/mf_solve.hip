@@ -1460,8 +1460,8 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
   HostSchedule nsched;  // allocation-free solves: the schedule built on the host
   if (opt.form < MF_FORM_AUTO || opt.form > MF_FORM_RESIDENT) return fail(MF_ERR_ARG, "solve: unknown form");
   {
-    // The grid is the occupancy of the streamed kernel (one block of 16
-    // waves per CU for both forms).  Unless the memory form is asked for,
+    // The grid is at most the occupancy of the streamed kernel (one block of
+    // 16 waves per CU for both forms).  Unless the memory form is asked for,
     // the resident-window kernel runs when every wave's share fits a
     // resident window (not for the rule policy's BiCGSTAB, which multiplies
     // p and s at different segment levels).
@@ -1471,7 +1471,19 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
                                : (mode == kAll64 ? (const void *)k_cg<kAll64, false>
                                   : mode == kPerSeg ? (const void *)k_cg<kPerSeg, false>
                                                     : (const void *)k_cg<kPerTile, false>);
-    sst = grid_for(kstream, opt.blocks, &G);
+    // Without an override, a small system gets about 1024 rows per block (at
+    // least 4 blocks; whole multiples of 8 from 8 on, for the XCD-contiguous
+    // block order), so the grid-wide sums and syncs span fewer blocks: on
+    // MI355X an FP64 CG iteration of a 4096-row system takes 10.1 us on 4
+    // blocks against 13.5 us on the full grid, and of a 32768-row one 11.1 us
+    // on 32 blocks against 14.4 us.  From about 256k rows on, the full grid.
+    int want = opt.blocks;
+    if (want <= 0) {
+      const int rows = t->tilem * 16;
+      want = std::max(4, (rows + 1023) / 1024);
+      if (want >= 8) want = (want + 7) / 8 * 8;
+    }
+    sst = grid_for(kstream, want, &G);
     if (sst != MF_OK) return sst;
     if (noalloc) {
       build_schedule(t, G * kWavesPerBlock, nsched);
@@ -1488,7 +1500,7 @@ mf_status run_persistent(mf_tiled *t, const double *d_b, double *d_x, const mf_s
                               : (mode == kAll64 ? (const void *)k_cg<kAll64, true>
                                  : mode == kPerSeg ? (const void *)k_cg<kPerSeg, true>
                                                    : (const void *)k_cg<kPerTile, true>);
-      sst = grid_for(kres, opt.blocks, &Gr);
+      sst = grid_for(kres, want, &Gr);
       if (sst != MF_OK) return sst;
       if (Gr == G) {
         kern = kres;
