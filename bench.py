@@ -216,10 +216,13 @@ def other_workload(args):
         res = {}
         for pol in ("errbound", "fp64"):
             for _ in range(args.warmup):
-                mf.batch_cg(tms, bs, nrhs=nrhs, tol=1e-10, policy=pol, batch_waves=waves)
+                mf.batch_cg(tms, bs, nrhs=nrhs, tol=1e-10, policy=pol, batch_waves=waves,
+                            refresh_ratio=args.refresh_ratio, refresh_every=args.refresh_every)
             tot = []
             for _ in range(args.steps):
-                _, rep = mf.batch_cg(tms, bs, nrhs=nrhs, tol=1e-10, policy=pol, batch_waves=waves)
+                _, rep = mf.batch_cg(tms, bs, nrhs=nrhs, tol=1e-10, policy=pol, batch_waves=waves,
+                                     refresh_ratio=args.refresh_ratio,
+                                     refresh_every=args.refresh_every)
                 tot.append(rep["total_ms"])
             per = np.asarray(rep["per_system_ms"])
             res[pol] = {"total_ms": round(float(np.mean(tot)), 3),
