@@ -212,8 +212,9 @@ bool regular_tilerows(const mf_tiled *t) {
 }
 
 int default_spmv_kernel(const mf_tiled *) {
-  // Measured on MI355X (3D 7-point 128^3): streamed 48 us at 2048 entries
-  // per wave, row-group 54 us; the streamed kernel also handles any sparsity.
+  // The streamed kernel handles any sparsity and was the faster of the two
+  // on the 3D 7-point 128^3 Laplacian on MI355X (mf_tune_spmv also times
+  // the row-group kernel on matrices with regular tile-rows).
   return MF_SPMV_STREAM;
 }
 
