@@ -210,7 +210,9 @@ mf_status mf_stream_copy(const double *d_src, double *d_dst, int64_t n,
 #define MF_POLICY_FP64 0    /* every tile FP64: the plain single-kernel CG */
 #define MF_POLICY_RULE 1    /* reference rule on |p_new - p_old| per segment */
 #define MF_POLICY_FIXED 2   /* use the levels set with mf_tile_set_precision */
-#define MF_POLICY_LEARNED 3 /* MLP over per-tile features (mf_policy_*) */
+#define MF_POLICY_LEARNED 3 /* MLP over per-tile features (mf_policy_*); the
+                               FP32 / FP16 / INT8 classes of a solver wave's
+                               tiles run at the most precise of them */
 #define MF_POLICY_ERRBOUND 4 /* analytic per-tile quantisation error bound */
 
 typedef struct mf_solve_opts {

@@ -239,8 +239,9 @@ def test_learned_policy_gains_over_rule(mf, oracle, policy, small_params):
     """What the trained network gains over the reference's rule: both runs
     reach the FP64-checked 1e-10; the rule's absolute thresholds send tiles
     to SKIP, lose definiteness and fall back to FP64, while the learned
-    policy keeps most tiles at INT8/FP32 to the end, moving at most 60% of
-    the FP64 payload bytes per SpMV and fewer bytes in total than the rule."""
+    policy keeps most tiles below FP64 to the end (a wave's FP32 / FP16 /
+    INT8 classes run at the most precise of them), moving at most 60% of the FP64 payload bytes per SpMV and
+    fewer bytes in total than the rule."""
     rp, ci, va, b = _system("lap3d_scaled32")
     n = len(rp) - 1
     tm = mf.TiledMatrix(rp, ci, va)
