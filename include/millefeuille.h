@@ -213,7 +213,9 @@ mf_status mf_stream_copy(const double *d_src, double *d_dst, int64_t n,
 #define MF_POLICY_LEARNED 3 /* MLP over per-tile features (mf_policy_*); the
                                FP32 / FP16 / INT8 classes of a solver wave's
                                tiles run at the most precise of them */
-#define MF_POLICY_ERRBOUND 4 /* analytic per-tile quantisation error bound */
+#define MF_POLICY_ERRBOUND 4 /* analytic per-tile quantisation error bound;
+                                the FP32 / FP16 / INT8 levels of a solver
+                                wave's tiles run at the most precise of them */
 
 typedef struct mf_solve_opts {
   double tol;           /* stop at ||b - A x||_2 <= tol * ||b||_2 (FP64) */

@@ -365,6 +365,8 @@ extern "C" mf_status mf_batch_cg_solve(mf_tiled **systems, int nsys, int nrhs, c
   int *d_tr = nullptr;
   MF_TRY(dalloc((void **)&d_tr, sizeof(int) * htr.size()));
   MF_HIP(hipMemcpyAsync(d_tr, htr.data(), sizeof(int) * htr.size(), hipMemcpyHostToDevice, st));
+  if (opt.policy == MF_POLICY_ERRBOUND)  // one level group per window
+    for (int i = 0; i < nsys; ++i) MF_TRY(wave_uniform_levels(systems[i], d_tr + (size_t)i * (nwv + 1), nwv, st));
   for (int i = 0; i < nsys; ++i) {
     hs[i].A = view_of(systems[i]);
     hs[i].wave_tr = d_tr + (size_t)i * (nwv + 1);

@@ -170,4 +170,7 @@ bool regular_tilerows(const mf_tiled *t);
 // Error-bound precision levels into the handle's current levels
 // (stream-ordered; stream is a hipStream_t).
 mf_status apply_errbound_levels(mf_tiled *t, void *stream);
+// The FP32 / FP16 / INT8 levels of each of nwaves waves' tiles (tile-rows
+// d_wave_tr[w] .. d_wave_tr[w+1], device) set to the most precise of them.
+mf_status wave_uniform_levels(mf_tiled *t, const int *d_wave_tr, int nwaves, void *stream);
 }  // namespace mf
