@@ -334,6 +334,22 @@ def main():
         f"rel(cpu)={rel_cpu:.3e} hist={rep['prec_hist']}")
     log(f"fp64: {fp64_ms:.2f} ms/solve (kernel {rep64['time_ms']:.2f} ms), it={rep64['iterations']} "
         f"rel(cpu)={rel64:.3e}")
+    # the other policies on the same system, timed the same way (not `value`)
+    others = {}
+    for pol in ("rule", "errbound"):
+        solve(pol)
+        t_o = []
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            r_o = solve(pol)
+            t_o.append(time.perf_counter() - t1)
+        ms_o = 1e3 * sum(t_o) / args.steps
+        others[pol] = {"ms": round(ms_o, 3), "kernel_ms": round(r_o["time_ms"], 3),
+                       "iterations": r_o["iterations"], "refreshes": r_o["refreshes"],
+                       "fell_back": r_o["fell_back"], "policy_used": r_o["policy_used"],
+                       "rel_residual_fp64_cpu": oracle.rel_residual(rp, ci, va, b, dx.get()[:n]),
+                       "prec_hist": r_o["prec_hist"]}
+        log(f"{pol}: {others[pol]}")
 
     # --- stand-alone SpMV roofline (all-FP64 payload) ----------------------
     xs = np.random.default_rng(0).uniform(-1, 1, n)
@@ -428,6 +444,9 @@ def main():
         "fp64": {"ms": round(fp64_ms, 3), "kernel_ms": round(rep64["time_ms"], 3),
                  "iterations": rep64["iterations"], "rel_residual_fp64_cpu": rel64},
         "speedup_vs_fp64": round(fp64_ms / mixed_ms, 3),
+        "rule": others["rule"],
+        "errbound": others["errbound"],
+        "speedup_vs_rule": round(others["rule"]["ms"] / mixed_ms, 3),
         "spmv": {"us_per_launch": round(t_spmv * 1e3, 2), "us_single_launch": round(t_single * 1e3, 2),
                  "bytes": bytes_spmv, "GBps": round(gbs, 1), "wave_nnz": wave_nnz, "tune": tune,
                  "kernel": tm.spmv_kernel_name(), "copy_GBps_measured": round(copy_gbs, 1)},
